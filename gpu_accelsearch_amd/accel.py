@@ -1,0 +1,400 @@
+"""Python face of the MI355X f-fdot acceleration search.
+
+Mirrors the reference's host interface (AccelSearch.cu and the headers under
+headers/) on top of libfdas.so.  Two layers:
+
+* ``AccelSearch`` -- the plan API: one call whitens a device-resident series and
+  runs every overlap-save step of the f-fdot search in fused HIP kernels.
+* reference-named functions (``createSubharmonicInfo``, ``realToComplexFFT``,
+  ``deredden``, ``generate_harmonics_ffdot``, ``perform_*`` ...) with the
+  reference's argument meaning, for code that drives the stages one by one.
+
+Device memory is plain ``DeviceArray`` allocations made by the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from ._native import check, lib
+
+
+# ----------------------------------------------------------------- device memory
+class DeviceArray:
+    """A device allocation with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(int(s) for s in np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        check(lib().fdas_malloc(C.byref(p), max(self.nbytes, 1)), "fdas_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        if a.nbytes:
+            check(lib().fdas_memcpy_htod(d.ptr, a.ctypes.data, a.nbytes), "fdas_memcpy_htod")
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            check(lib().fdas_memcpy_dtoh(out.ctypes.data, self.ptr, self.nbytes), "fdas_memcpy_dtoh")
+        return out
+
+    def zero(self) -> None:
+        check(lib().fdas_memset(self.ptr, 0, self.nbytes), "fdas_memset")
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().fdas_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    check(lib().fdas_device_synchronize(), "fdas_device_synchronize")
+
+
+# ----------------------------------------------------------------- plan API
+@dataclass
+class SearchParams:
+    """Parameters of one DM-trial search (AccelSearch.cu:16-40, 52-58, 90-101)."""
+    n: int
+    dt: float = 6.4e-5
+    zmax: int = 200
+    numharm: int = 8
+    fftlen: int = 8192
+    uselen: int = 0
+    precision: str = "fp32"
+    chunk_steps: int = 0
+    max_candidates: int = 0
+    powcut: tuple = field(default_factory=lambda: (0.0,) * N.MAX_STAGES)
+
+    def to_c(self) -> N.FdasParams:
+        p = N.FdasParams()
+        p.n, p.dt, p.zmax, p.numharm = self.n, self.dt, self.zmax, self.numharm
+        p.fftlen, p.uselen, p.precision = self.fftlen, self.uselen, N.PREC[self.precision]
+        p.chunk_steps, p.max_candidates = self.chunk_steps, self.max_candidates
+        pc = list(self.powcut) + [0.0] * N.MAX_STAGES
+        for i in range(N.MAX_STAGES):
+            p.powcut[i] = float(pc[i])
+        return p
+
+
+CANDIDATE_DTYPE = np.dtype([("step", np.int32), ("z_index", np.int32), ("r_index", np.int32),
+                            ("stage", np.int32), ("power", np.float32), ("reserved", np.float32)])
+
+
+class AccelSearch:
+    """Fused f-fdot search of one series shape (templates stay on the device)."""
+
+    def __init__(self, params: SearchParams):
+        self.params = params
+        h = C.c_void_p()
+        cp = params.to_c()
+        check(lib().fdas_plan_create(C.byref(cp), C.byref(h)), "fdas_plan_create")
+        self.handle = h.value
+        info = N.FdasPlanInfo()
+        check(lib().fdas_plan_get_info(self.handle, C.byref(info)), "fdas_plan_get_info")
+        self.info = info
+        self.uselen, self.rlo, self.nsteps = info.uselen, info.rlo, info.nsteps
+        self.numz, self.nbins, self.cells = info.numz, info.nbins, info.cells
+        self.harmonics = [(info.harm_cur[h], info.harm_max[h], info.fftlen[h], info.numzs[h])
+                          for h in range(info.numharm)]
+        self._series = None
+
+    # -- stages
+    def prepare(self, series) -> None:
+        """R2C FFT and whitening of a host array or DeviceArray of n float32."""
+        if isinstance(series, DeviceArray):
+            d = series
+        else:
+            x = np.ascontiguousarray(series, dtype=np.float32)
+            if x.shape != (self.params.n,):
+                raise ValueError(f"series must have shape ({self.params.n},), got {x.shape}")
+            d = DeviceArray.from_host(x)
+        self._series = d
+        check(lib().fdas_prepare(self.handle, d.ptr), "fdas_prepare")
+
+    def run_steps(self, step0: int, nsteps: int, out: DeviceArray | None = None):
+        """Steps [step0, step0+nsteps); returns the summed planes on the host
+        (shape [nsteps, numz, uselen]) unless ``out`` (a DeviceArray) is given."""
+        dev = out if out is not None else DeviceArray((nsteps, self.numz, self.uselen), np.float32)
+        check(lib().fdas_run_steps(self.handle, step0, nsteps, dev.ptr), "fdas_run_steps")
+        self.synchronize()
+        return dev if out is not None else dev.to_host()
+
+    def search(self, series=None) -> None:
+        """One DM trial: prepare + every step (planes stay on the device)."""
+        if series is not None:
+            self.prepare(series)
+        check(lib().fdas_run_steps(self.handle, 0, self.nsteps, None), "fdas_run_steps")
+
+    def synchronize(self) -> None:
+        check(lib().fdas_synchronize(self.handle), "fdas_synchronize")
+
+    def enable_timing(self, on: bool = True) -> None:
+        check(lib().fdas_plan_enable_timing(self.handle, int(on)), "fdas_plan_enable_timing")
+
+    def timing(self) -> dict:
+        """{category: (launches, total ms)} from HIP events on the plan's stream."""
+        t = N.FdasTiming()
+        check(lib().fdas_plan_get_timing(self.handle, C.byref(t)), "fdas_plan_get_timing")
+        return {name: (t.launches[i], t.ms[i]) for i, name in enumerate(N.KERNEL_CATEGORIES)}
+
+    def spectrum(self) -> np.ndarray:
+        d = DeviceArray((self.nbins,), np.complex64)
+        check(lib().fdas_get_spectrum(self.handle, d.ptr), "fdas_get_spectrum")
+        self.synchronize()
+        return d.to_host()
+
+    def candidates(self) -> np.ndarray:
+        n, ov = C.c_int(), C.c_int()
+        check(lib().fdas_get_candidates(self.handle, None, 0, C.byref(n), C.byref(ov)),
+              "fdas_get_candidates")
+        out = np.zeros(n.value, CANDIDATE_DTYPE)
+        if n.value:
+            check(lib().fdas_get_candidates(
+                self.handle, out.ctypes.data_as(C.POINTER(N.FdasCandidate)), n.value,
+                C.byref(n), C.byref(ov)), "fdas_get_candidates")
+        self.overflowed = bool(ov.value)
+        return out
+
+    def candidate_rz(self, c: np.ndarray):
+        """Fourier frequency r and drift z of candidates."""
+        r = self.rlo + c["step"].astype(np.float64) * (self.uselen // 2) + 0.5 * c["r_index"]
+        z = -self.params.zmax + 2.0 * c["z_index"]
+        return r, z
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().fdas_plan_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stage_powcut(sigma: float, numharm: int, numindep: float = 1.0) -> float:
+    """Summed power whose false-alarm probability over ``numindep`` trials equals
+    the Gaussian tail beyond ``sigma``.  Each normalised power is Exp(1) under
+    noise, so the sum of ``numharm`` of them is Gamma(numharm, 1)."""
+    from scipy.special import gammainccinv, ndtr
+    p = (1.0 - ndtr(sigma)) / max(numindep, 1.0)
+    return float(gammainccinv(numharm, p))
+
+
+# ----------------------------------------------------------------- reference-named API
+def fdas_set_uselen(u: int) -> None:
+    if lib().fdas_set_uselen(int(u)) != 0:
+        raise ValueError(N.last_error())
+
+
+def calc_required_z(f: float, zfull: float) -> int:
+    return lib().calc_required_z(f, zfull)
+
+
+def calc_required_r(f: float, rfull: float) -> float:
+    return lib().calc_required_r(f, rfull)
+
+
+def index_from_r(r: float, rlo: float) -> int:
+    return lib().index_from_r(r, rlo)
+
+
+def index_from_z(z: float, zlo: float) -> int:
+    return lib().index_from_z(z, zlo)
+
+
+def getTotalHarmonics(stages: int) -> int:
+    return lib().getTotalHarmonics(stages)
+
+
+def z_response_halfwidth(z: float) -> int:
+    return lib().z_response_halfwidth(z)
+
+
+def calculateFFTLength(numharm: int, harmnum: int, max_zfull: int) -> int:
+    return lib().calculateFFTLength(numharm, harmnum, max_zfull)
+
+
+class SubharmonicBank:
+    """createSubharmonicInfo() result: per-harmonic device template banks."""
+
+    def __init__(self, stages: int, zhi: int):
+        self.stages = stages
+        self.total = getTotalHarmonics(stages)
+        p = lib().createSubharmonicInfo(stages, zhi)
+        if not p:
+            raise N.FdasError(f"createSubharmonicInfo: {N.last_error()}")
+        self.ptr = p
+        self.entries = [p[h] for h in range(self.total)]
+
+    def bank(self, h: int) -> np.ndarray:
+        e = self.ptr[h]
+        out = np.empty((e.totalKernelsTemplates, e.kernelFFTLength), np.complex64)
+        check(lib().fdas_memcpy_dtoh(out.ctypes.data, e.kernels_data, out.nbytes), "bank copy")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().freeSubharmonicInfo(self.ptr, self.total)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def createSubharmonicInfo(numHarmonicStages: int, zhi: int) -> SubharmonicBank:
+    return SubharmonicBank(numHarmonicStages, zhi)
+
+
+def realToComplexFFT(x: np.ndarray) -> np.ndarray:
+    """R2C transform of a host float32 series -> n/2+1 complex64 bins."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = C.c_void_p()
+    n = lib().realToComplexFFT(x.ctypes.data, len(x), C.byref(out), 1)
+    if n == 0:
+        raise N.FdasError(f"realToComplexFFT: {N.last_error()}")
+    buf = (C.c_float * (2 * n)).from_address(out.value)
+    res = np.frombuffer(buf, np.float32).copy().view(np.complex64)
+    C.CDLL(None).free(C.c_void_p(out.value))
+    return res
+
+
+def _c2c(d_in: DeviceArray, n: int, batch: int, inverse: bool) -> DeviceArray:
+    out = C.c_void_p()
+    fn = lib().inverseComplexToComplexFFT if inverse else lib().complexToComplexFFT
+    if fn(d_in.ptr, n, C.byref(out), batch) == 0:
+        raise N.FdasError(N.last_error())
+    d = DeviceArray.__new__(DeviceArray)
+    d.shape, d.dtype, d.ptr = (batch, n), np.dtype(np.complex64), out.value
+    d.nbytes = batch * n * 8
+    return d
+
+
+def complexToComplexFFT(d_in: DeviceArray, n: int, batch: int = 1) -> DeviceArray:
+    return _c2c(d_in, n, batch, False)
+
+
+def inverseComplexToComplexFFT(d_in: DeviceArray, n: int, batch: int = 1) -> DeviceArray:
+    return _c2c(d_in, n, batch, True)
+
+
+def deredden(spec: np.ndarray, numOfBins: int | None = None) -> np.ndarray:
+    """Whitening of bins [0, numOfBins) of a complex64 spectrum (returns a copy)."""
+    a = np.ascontiguousarray(spec, dtype=np.complex64).copy()
+    nb = len(a) if numOfBins is None else int(numOfBins)
+    lib().fdas_clear_error()
+    lib().deredden(a.ctypes.data, nb)
+    if N.last_error():
+        raise N.FdasError(N.last_error())
+    return a
+
+
+def harmonic_fraction_order(stages: int):
+    out = []
+    for s in range(stages):
+        mx = 1 if s == 0 else 1 << s
+        out += [(c, mx) for c in range(1, mx + 1, 2)]
+    return out
+
+
+def generate_harmonics_ffdot(bank: SubharmonicBank, fullrlo: float, fullrhi: float, zlo: float):
+    """Per-harmonic ffdotInfo of one step and the bin offsets (source/bpd_search.cu:72)."""
+    H = bank.total
+    info = (N.FfdotInfo * H)()
+    bo = (C.c_ushort * H)()
+    lib().generate_harmonics_ffdot(H, bank.stages, bank.ptr, fullrlo, fullrhi, zlo, info, bo)
+    return info, bo
+
+
+def step_bounds(startr: float, uselen: int):
+    """startr -> (startr, lastr) as in AccelSearch.cu:211-212."""
+    nextr = startr + uselen * 0.5
+    return startr, nextr - 0.5
+
+
+class StagePipeline:
+    """The reference's per-step stage sequence (AccelSearch.cu:217-241) on device
+    buffers laid out as there: harmonics concatenated at dataStartIdx /
+    convolutionHarmonicIdx, and the fundamental power plane [numzs][numrs]."""
+
+    def __init__(self, bank: SubharmonicBank):
+        self.bank = bank
+        H = bank.total
+        self.H = H
+        L = [bank.entries[h].kernelFFTLength for h in range(H)]
+        Z = [bank.entries[h].totalKernelsTemplates for h in range(H)]
+        self.dataStartIdx = (C.c_long * H)(*np.concatenate([[0], np.cumsum(L)[:-1]]).astype(int))
+        self.convIdx = (C.c_long * H)(*np.concatenate(
+            [[0], np.cumsum(np.array(L) * np.array(Z))[:-1]]).astype(int))
+        self.total_fft = int(sum(L))
+        self.total_conv = int(np.sum(np.array(L) * np.array(Z)))
+        self.d_data = DeviceArray((self.total_fft,), np.complex64)
+        self.d_fft = DeviceArray((self.total_fft,), np.complex64)
+        self.d_conv = DeviceArray((self.total_conv,), np.complex64)
+        self.d_ifft = DeviceArray((self.total_conv,), np.complex64)
+        lib().initialize_cuda_plans(H, bank.ptr)
+
+    def step(self, spectrum: np.ndarray, startr: float, zlo: float, uselen: int,
+             nbins: int, stage: str = "summed") -> np.ndarray:
+        """One step from a host whitened spectrum; returns the power plane."""
+        H = self.H
+        s0, s1 = step_bounds(startr, uselen)
+        info, bo = generate_harmonics_ffdot(self.bank, s0, s1, zlo)
+        host = np.zeros(self.total_fft, np.complex64)
+        spec = np.ascontiguousarray(spectrum, dtype=np.complex64)
+        lib().fdas_clear_error()
+        lib().generate_harmonic_ffdot_data(H, self.total_fft, info, bo, self.dataStartIdx,
+                                           spec.ctypes.data, nbins, 0, host.ctypes.data)
+        if N.last_error():
+            raise N.FdasError(N.last_error())
+        check(lib().fdas_memcpy_htod(self.d_data.ptr, host.ctypes.data, host.nbytes), "htod")
+        lib().perform_fft_harmonics(H, self.d_data.ptr, self.dataStartIdx, self.d_fft.ptr)
+        lib().perform_convolution(H, self.d_fft.ptr, info, self.bank.ptr, self.dataStartIdx,
+                                  self.d_conv.ptr, self.convIdx)
+        lib().perform_inversefft(H, self.d_conv.ptr, self.convIdx, self.d_ifft.ptr)
+        numz, numrs = info[0].numzs, info[0].numrs
+        d_pow = DeviceArray((numz, numrs), np.float32)
+        lib().generate_fundamental_powers(H, self.d_ifft.ptr, self.convIdx, info, self.bank.ptr,
+                                          d_pow.ptr)
+        if stage == "summed":
+            lib().perform_harmonic_summing(H, self.d_ifft.ptr, self.convIdx, info, self.bank.ptr,
+                                           d_pow.ptr, int(startr), None, None, 0)
+        synchronize()
+        if N.last_error():
+            raise N.FdasError(N.last_error())
+        return d_pow.to_host()
+
+    def close(self) -> None:
+        lib().free_cuda_plans(self.H)
+
+
+def dm_trials_per_second(seconds_per_trial: float) -> float:
+    return 1.0 / seconds_per_trial if seconds_per_trial > 0 else math.inf

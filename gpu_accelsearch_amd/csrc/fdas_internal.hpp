@@ -1,0 +1,135 @@
+// fdas_internal.hpp -- declarations shared by the HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fdas.h"
+#include "fdas_common.hpp"
+
+namespace fdas {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char* fmt, ...);
+const char* last_error();
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define FDAS_HIP(call)                                                         \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) return ::fdas::hip_fail(e_, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define FDAS_TRY(call)                  \
+    do {                                \
+        int rc_ = (call);               \
+        if (rc_ != FDAS_OK) return rc_; \
+    } while (0)
+
+int check_launch(const char* what);
+
+// ---------------------------------------------------------------- FFT
+// Threads per workgroup used for a length-N row transform (N = 64..16384).
+int fft_threads(int n);
+bool fft_row_supported(long n);
+// Device twiddle table of the row transform of length n (cached per device).
+const cplx* fft_twiddles(int n, int* err);
+
+// Row transforms: out[row*out_stride + k] = scale * DFT_dir(in[row*in_stride + .])[k]
+// (times exp(dir 2 pi i (row*k mod tw_m)/tw_m) when tw_m > 0).  In place allowed.
+int fft_rows(const cplx* in, cplx* out, int n, long nrows, long in_stride, long out_stride,
+             int dir, float scale, long tw_m, hipStream_t s);
+// Transpose in[rows][cols] -> out[cols][rows].
+int transpose(const cplx* in, cplx* out, long rows, long cols, hipStream_t s);
+// Complex transform of any power-of-two length 64..2^28 (four-step beyond 16384).
+// work: 2*m complex scratch (unused for m <= 16384).  out may equal in.
+int fft_c2c(const cplx* in, cplx* out, long m, int dir, cplx* work, hipStream_t s);
+// Real-to-complex transform of n (power of two) floats -> n/2+1 bins.
+// work: 3*(n/2) complex scratch.
+int fft_r2c(const float* in, cplx* out, long n, cplx* work, hipStream_t s);
+size_t fft_r2c_work_elems(long n);
+
+// ---------------------------------------------------------------- whitening
+struct WhitenSchedule {
+    unsigned nbins = 0;
+    int nblocks = 0;            // median blocks
+    int* d_start = nullptr;     // block start bin
+    int* d_len = nullptr;       // block length
+    float* d_median = nullptr;  // block medians (filled on the device)
+    int nsegs = 0;              // normalisation segments = nblocks + 1
+    int* d_seg_lo = nullptr;    // first bin of segment
+    int* d_seg_hi = nullptr;    // one past the last bin
+};
+int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws);
+void whiten_schedule_destroy(WhitenSchedule* ws);
+int whiten(cplx* d_spec, const WhitenSchedule& ws, hipStream_t s);
+
+// ---------------------------------------------------------------- templates
+// Host-side geometry of the reference (source/bpd_subharmonics.cu).
+int h_nearest_int(double x);
+int h_calc_required_z(double f, double zfull);
+double h_calc_required_r(double f, double rfull);
+int h_z_response_halfwidth(double z);
+long long h_next_pow2(long long x);
+int h_fast_log2(int n);
+int h_fft_length(int uselen, int numharm, int harmnum, int zmax);
+int h_max_uselen(int fftlen, int zmax);
+void h_harmonic_order(int numharm, int* cur, int* max);
+
+// Fill bank[k*L .. (k+1)*L) with the FFT'd template rows z_k = -zmax_h + 2k,
+// k < nz (FP64 generation, FP32 storage, FP32 FFT on the device).
+int build_template_bank(cplx* d_bank, int zmax_h, int nz, int L, hipStream_t s);
+
+// ---------------------------------------------------------------- search
+struct SegDesc {       // one (step, harmonic) segment of the overlap-save loop
+    int lobin;         // first spectrum bin fetched
+    int numdata;       // bins normalised (median taken over these)
+    long long out;     // element offset of its L/2 transformed amplitudes
+};
+
+struct HarmSumArgs {
+    int H;                        // harmonics summed (1 = fundamental only)
+    int U;                        // fundamental row width (uselen)
+    int zlo0;                     // fundamental zlo (= -zmax)
+    int c[MAX_HARM];              // harmonic fraction numerator
+    int lm[MAX_HARM];             // log2 of the denominator
+    int zlo[MAX_HARM];
+    int Z[MAX_HARM];
+    int W[MAX_HARM];              // sub-plane row width
+    long long sub_off[MAX_HARM];  // element offset of sub-plane h in the sub buffer
+    long long sub_stride[MAX_HARM];  // elements per step (Z*W)
+    float powcut[5];
+    int cand_cap;
+};
+
+// Normalise + transform every segment in desc[0..nseg) (all of FFT length L);
+// segment d lands at d_seg + (d.out - out_base).
+int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_desc, int nseg,
+                        cplx* d_seg, long long out_base, hipStream_t s);
+
+// Fused correlation + inverse FFT + power for one sub-harmonic:
+// out[(s*Z + z)*W + i] = |IFFT(A_s * conj(K_z))[off + i]|^2 / L^2, i < W.
+int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
+                    long long seg_off, int nsteps, int off, int W, float* d_out,
+                    hipStream_t s);
+
+// Fundamental version with the harmonic sum, stage thresholds and plane write.
+int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
+                     int step0, int nsteps, int off, const int* d_fullrlo, const int* d_rlo_h,
+                     const float* d_sub, const HarmSumArgs& hs, float* d_plane,
+                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s);
+
+// Stage-API kernels (unfused, reference data layout).
+int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,
+                    hipStream_t s);
+int launch_powers(const cplx* d_ifft, int Z, int numrs, int L, float scale, float* d_out,
+                  hipStream_t s);
+int launch_add_powers(int Z0, int numrs0, int Lh, int zlo0, int zlo_h, int rlo_h, double f,
+                      const cplx* d_ifft_h, float scale, float* d_powers, int fullrlo,
+                      hipStream_t s);
+int launch_normalize_spread(const cplx* d_amps, int numdata, int nhalf, cplx* d_out,
+                            float* d_median_out, hipStream_t s);
+
+}  // namespace fdas

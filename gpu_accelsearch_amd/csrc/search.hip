@@ -1,0 +1,458 @@
+// search.hip -- the overlap-save f-fdot loop on the GPU.
+//
+// Fused path (one launch per size class and harmonic, covering a batch of steps):
+//   seg_prep_kernel   per (step, harmonic): fetch L/2 bins, exact lower median of
+//                     the first numdata powers, scale by the fast inverse square
+//                     root of median/ln2, forward FFT of length L/2.  The 2x spread
+//                     of source/bpd_search.cu:56-67,181 makes the length-L forward
+//                     transform two copies of this one, so only L/2 is stored.
+//   corr_sub_kernel   per (template, step) of a sub-harmonic: multiply by the
+//                     conjugate template, inverse FFT of length L in LDS, |y|^2/L^2
+//                     of the kept columns -> sub-harmonic power plane.
+//   corr_fund_kernel  the same for the fundamental, then the harmonic sum of
+//                     source/bpd_gpu_kernels.cu:119-170 in the reference order,
+//                     per-stage thresholds, and one write of the summed plane.
+// The complex f-fdot plane is never written to memory.
+//
+// Stage path (reference data layout, source/bpd_search.cu:228-360): convolve,
+// powers and add_powers kernels plus the normalise-and-spread kernel used by the
+// reference-named entry points.
+#include <cmath>
+
+#include "fdas_internal.hpp"
+#include "fft_block.hpp"
+
+namespace fdas {
+
+template <int N>
+struct FftCfgS;
+template <> struct FftCfgS<512>   { static constexpr int T = 64; };
+template <> struct FftCfgS<1024>  { static constexpr int T = 64; };
+template <> struct FftCfgS<2048>  { static constexpr int T = 128; };
+template <> struct FftCfgS<4096>  { static constexpr int T = 256; };
+template <> struct FftCfgS<8192>  { static constexpr int T = 256; };
+template <> struct FftCfgS<16384> { static constexpr int T = 512; };
+
+namespace {
+constexpr double kLn2 = 0.69314718055994530942;
+}
+
+__device__ __forceinline__ float quake_rsqrt_s(float x)
+{
+#pragma clang fp contract(off)
+    int i = __float_as_int(x);
+    i = 0x5f3759df - (i >> 1);
+    const float y = __int_as_float(i);
+    float h = 0.5f * x;
+    h = h * y;
+    h = h * y;
+    return y * (1.5f - h);
+}
+
+__device__ __forceinline__ float power_nc(cplx a)
+{
+#pragma clang fp contract(off)
+    const float u = a.x * a.x;
+    const float v = a.y * a.y;
+    return u + v;
+}
+
+__device__ __forceinline__ cplx scale_nc(cplx a, float c)
+{
+#pragma clang fp contract(off)
+    return mk(a.x * c, a.y * c);
+}
+
+// One 8-bit digit step of a workgroup radix select over non-negative float bit
+// patterns.  Counts the candidates matching (bits & mask) == prefix by digit,
+// then wavefront 0 finds the digit holding rank k.  Returns the digit and the
+// rank within it through sel[].  hist has 256 entries.
+template <int T>
+__device__ __forceinline__ void select_digit(unsigned* hist, int* sel, int k, int t)
+{
+    __syncthreads();
+    if (t < 64) {
+        const unsigned c0 = hist[4 * t], c1 = hist[4 * t + 1], c2 = hist[4 * t + 2],
+                       c3 = hist[4 * t + 3];
+        const unsigned sum = c0 + c1 + c2 + c3;
+        unsigned incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned y = __shfl_up(incl, o, 64);
+            if (t >= o) incl += y;
+        }
+        const unsigned excl = incl - sum;
+        if ((unsigned)k >= excl && (unsigned)k < incl) {
+            unsigned r = (unsigned)k - excl;
+            int d = 4 * t;
+            if (r >= c0) {
+                r -= c0; d++;
+                if (r >= c1) {
+                    r -= c1; d++;
+                    if (r >= c2) { r -= c2; d++; }
+                }
+            }
+            sel[0] = d;
+            sel[1] = (int)r;
+        }
+    }
+    __syncthreads();
+}
+
+// ------------------------------------------------------------------ fused path
+template <int NH>
+__global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
+    const cplx* __restrict__ spec, long nbins, const SegDesc* __restrict__ desc,
+    cplx* __restrict__ seg, long long out_base, const cplx* __restrict__ tw)
+{
+    constexpr int T = FftCfgS<NH>::T;
+    using F = BlockFFT<NH, T, -1>;
+    __shared__ cplx lds[F::LDS_ELEMS];
+    __shared__ unsigned hist[256];
+    __shared__ int sel[2];
+    const SegDesc d = desc[blockIdx.x];
+    const int t = threadIdx.x;
+    cplx v[F::E];
+    unsigned pb[F::E];
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) {
+        const int n = F::in_index(t, i);
+        const long b = (long)d.lobin + n;
+        v[i] = (b >= 0 && b < nbins) ? spec[b] : mk(0.f, 0.f);
+        pb[i] = __float_as_uint(power_nc(v[i]));
+    }
+    // exact lower median: rank (numdata-1)/2 among the first numdata powers
+    unsigned prefix = 0, mask = 0;
+    int k = (d.numdata - 1) / 2;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int q = t; q < 256; q += T) hist[q] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < F::E; ++i) {
+            const int n = F::in_index(t, i);
+            if (n < d.numdata && (pb[i] & mask) == prefix)
+                atomicAdd(&hist[(pb[i] >> shift) & 255u], 1u);
+        }
+        select_digit<T>(hist, sel, k, t);
+        prefix |= (unsigned)sel[0] << shift;
+        mask |= 255u << shift;
+        k = sel[1];
+    }
+    const float med = __uint_as_float(prefix);
+    const float c = quake_rsqrt_s((float)((double)med / kLn2));
+#pragma unroll
+    for (int i = 0; i < F::E; ++i)
+        if (F::in_index(t, i) < d.numdata) v[i] = scale_nc(v[i], c);
+    F::run(v, lds, tw, t);
+    cplx* o = seg + (d.out - out_base);
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) o[F::out_index(t, i)] = v[i];
+}
+
+// Shared body: y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.
+template <int L>
+__device__ __forceinline__ void corr_ifft(const cplx* __restrict__ K, const cplx* __restrict__ A,
+                                          const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
+{
+    constexpr int T = FftCfgS<L>::T;
+    using F = BlockFFT<L, T, 1>;
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) {
+        const int n = F::in_index(t, i);
+        v[i] = cmulc(A[n & (L / 2 - 1)], K[n]);
+    }
+    F::run(v, lds, tw, t);
+}
+
+template <int L>
+__global__ __launch_bounds__(FftCfgS<L>::T) void corr_sub_kernel(
+    const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
+    long long seg_off, int off, int W, float* __restrict__ out, const cplx* __restrict__ tw,
+    float scale)
+{
+    constexpr int T = FftCfgS<L>::T;
+    using F = BlockFFT<L, T, 1>;
+    __shared__ cplx lds[F::LDS_ELEMS];
+    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+    cplx v[F::E];
+    corr_ifft<L>(bank + (size_t)z * L, seg + s * seg_stride + seg_off, tw, lds, v, t);
+    float* o = out + ((size_t)s * Z + z) * W;
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) {
+        const int col = F::out_index(t, i) - off;
+        if (col >= 0 && col < W) o[col] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(FftCfgS<L>::T) void corr_fund_kernel(
+    const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
+    int step0, int off, const int* __restrict__ fullrlo, const int* __restrict__ rlo_h,
+    const float* __restrict__ sub, HarmSumArgs hs, float* __restrict__ plane,
+    fdas_candidate* __restrict__ cands, int* __restrict__ ncand, const cplx* __restrict__ tw,
+    float scale)
+{
+    constexpr int T = FftCfgS<L>::T;
+    using F = BlockFFT<L, T, 1>;
+    __shared__ cplx lds[F::LDS_ELEMS];
+    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+    const int gstep = step0 + s;
+    cplx v[F::E];
+    corr_ifft<L>(bank + (size_t)z * L, seg + s * seg_stride, tw, lds, v, t);
+
+    const long long q0 = 2LL * fullrlo[gstep];
+    const int zz = hs.zlo0 + DZ * z;
+    const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
+    float* o = plane + ((size_t)s * Z + z) * hs.U;
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) {
+        const int col = F::out_index(t, i) - off;
+        if (col < 0 || col >= hs.U) continue;
+        float acc = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+        int stage = 0;
+        if (acc > hs.powcut[0]) {
+            const int idx = atomicAdd(ncand, 1);
+            if (idx < hs.cand_cap) {
+                fdas_candidate c;
+                c.step = gstep; c.z_index = z; c.r_index = col; c.stage = 0; c.power = acc;
+                c.reserved = 0.f;
+                cands[idx] = c;
+            }
+        }
+        const long long q = q0 + col;
+        for (int h = 1; h < hs.H; ++h) {
+            // subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm, rr = fullrlo + col/2
+            const int lm = hs.lm[h];
+            const long long sr2 = (q * hs.c[h] + (1LL << (lm - 1))) >> lm;
+            const int rind = (int)(sr2 - 2LL * rl[h]);
+            // subz = (int)(zz f) truncates toward zero; zind = (subz - zlo_h) / 2
+            const int subz = (zz * hs.c[h]) / (1 << lm);
+            const int zind = (subz - hs.zlo[h]) >> 1;
+            acc += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+            if (((h + 1) & h) == 0) {
+                ++stage;
+                if (acc > hs.powcut[stage]) {
+                    const int idx = atomicAdd(ncand, 1);
+                    if (idx < hs.cand_cap) {
+                        fdas_candidate c;
+                        c.step = gstep; c.z_index = z; c.r_index = col; c.stage = stage;
+                        c.power = acc; c.reserved = 0.f;
+                        cands[idx] = c;
+                    }
+                }
+            }
+        }
+        o[col] = acc;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int NH>
+static int seg_prep_t(const cplx* spec, long nbins, const SegDesc* desc, int nseg, cplx* seg,
+                      long long out_base, hipStream_t s)
+{
+    int err = FDAS_OK;
+    const cplx* tw = fft_twiddles(NH, &err);
+    if (!tw) return err;
+    if (fft_threads(NH) != FftCfgS<NH>::T) {
+        set_error("thread count mismatch for length %d", NH);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(seg_prep_kernel<NH>, dim3(nseg), dim3(FftCfgS<NH>::T), 0, s, spec, nbins,
+                       desc, seg, out_base, tw);
+    return check_launch("seg_prep_kernel");
+}
+
+int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_desc, int nseg,
+                        cplx* d_seg, long long out_base, hipStream_t s)
+{
+    if (nseg <= 0) return FDAS_OK;
+    switch (L / 2) {
+        case 512: return seg_prep_t<512>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 1024: return seg_prep_t<1024>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 2048: return seg_prep_t<2048>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 4096: return seg_prep_t<4096>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 8192: return seg_prep_t<8192>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        default:
+            set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
+            return FDAS_ERR_UNSUPPORTED;
+    }
+}
+
+template <int L>
+static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride,
+                      long long seg_off, int nsteps, int off, int W, float* out, hipStream_t s)
+{
+    int err = FDAS_OK;
+    const cplx* tw = fft_twiddles(L, &err);
+    if (!tw) return err;
+    const float scale = (float)(1.0 / ((double)L * (double)L));
+    hipLaunchKernelGGL(corr_sub_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
+                       seg, seg_stride, seg_off, off, W, out, tw, scale);
+    return check_launch("corr_sub_kernel");
+}
+
+int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
+                    long long seg_off, int nsteps, int off, int W, float* d_out, hipStream_t s)
+{
+    if (nsteps <= 0) return FDAS_OK;
+    switch (L) {
+        case 1024: return corr_sub_t<1024>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 2048: return corr_sub_t<2048>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 4096: return corr_sub_t<4096>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 8192: return corr_sub_t<8192>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 16384: return corr_sub_t<16384>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        default:
+            set_error("fused correlation length %d unsupported", L);
+            return FDAS_ERR_UNSUPPORTED;
+    }
+}
+
+template <int L>
+static int corr_fund_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride, int step0,
+                       int nsteps, int off, const int* fullrlo, const int* rlo_h, const float* sub,
+                       const HarmSumArgs& hs, float* plane, fdas_candidate* cands, int* ncand,
+                       hipStream_t s)
+{
+    int err = FDAS_OK;
+    const cplx* tw = fft_twiddles(L, &err);
+    if (!tw) return err;
+    const float scale = (float)(1.0 / ((double)L * (double)L));
+    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
+                       seg, seg_stride, step0, off, fullrlo, rlo_h, sub, hs, plane, cands, ncand,
+                       tw, scale);
+    return check_launch("corr_fund_kernel");
+}
+
+int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
+                     int step0, int nsteps, int off, const int* d_fullrlo, const int* d_rlo_h,
+                     const float* d_sub, const HarmSumArgs& hs, float* d_plane,
+                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
+{
+    if (nsteps <= 0) return FDAS_OK;
+    switch (L) {
+        case 1024: return corr_fund_t<1024>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        case 2048: return corr_fund_t<2048>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        case 4096: return corr_fund_t<4096>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        case 8192: return corr_fund_t<8192>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        case 16384: return corr_fund_t<16384>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        default:
+            set_error("fused correlation length %d unsupported", L);
+            return FDAS_ERR_UNSUPPORTED;
+    }
+}
+
+// ------------------------------------------------------------------ stage path
+// source/bpd_gpu_kernels.cu:39-81: out[z][k] = X[k] * conj(K_z[k])
+__global__ __launch_bounds__(256) void convolve_kernel(const cplx* __restrict__ x,
+                                                       const cplx* __restrict__ bank, int L,
+                                                       cplx* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (k >= L) return;
+    const size_t idx = (size_t)z * L + k;
+    out[idx] = cmulc(x[k], bank[idx]);
+}
+
+// source/bpd_gpu_kernels.cu:84-116: P[z][i] = |y[z*L + i]|^2 * scale, i < numrs
+__global__ __launch_bounds__(256) void powers_kernel(const cplx* __restrict__ y, int numrs, int L,
+                                                     float scale, float* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    if (i >= numrs) return;
+    const cplx a = y[(size_t)z * L + i];
+    out[(size_t)z * numrs + i] = (a.x * a.x + a.y * a.y) * scale;
+}
+
+// source/bpd_gpu_kernels.cu:119-170 with its FP64 index arithmetic.
+__global__ __launch_bounds__(256) void add_powers_kernel(int Z0, int numrs0, int Lh, int zlo0,
+                                                         int zlo_h, int rlo_h, double f,
+                                                         const cplx* __restrict__ y, float scale,
+                                                         float* __restrict__ P, int fullrlo)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= numrs0) return;
+    const double rr = fullrlo + i * DR;
+    const double subr = (int)(RDR * rr * f + 0.5) * DR;
+    const unsigned short rind = (unsigned short)((subr - rlo_h) * RDR);
+    const int zz = zlo0 + j * DZ;
+    const int subz = (int)(zz * f);
+    const int zind = (int)((subz - zlo_h) * RDZ);
+    const cplx a = y[(size_t)zind * Lh + rind];
+    P[(size_t)j * numrs0 + i] += (a.x * a.x + a.y * a.y) * scale;
+}
+
+// source/bpd_search.cu:150-181 for one harmonic: median of the first numdata
+// powers, scale those amplitudes, spread into out[2i] (odd entries zero),
+// i < nhalf.  Single workgroup; amps holds max(numdata, nhalf) values.
+__global__ __launch_bounds__(1024) void normalize_spread_kernel(const cplx* __restrict__ amps,
+                                                                int numdata, int nhalf,
+                                                                cplx* __restrict__ out,
+                                                                float* __restrict__ med_out)
+{
+    constexpr int T = 1024;
+    __shared__ unsigned hist[256];
+    __shared__ int sel[2];
+    const int t = threadIdx.x;
+    unsigned prefix = 0, mask = 0;
+    int k = (numdata - 1) / 2;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int q = t; q < 256; q += T) hist[q] = 0;
+        __syncthreads();
+        for (int n = t; n < numdata; n += T) {
+            const unsigned b = __float_as_uint(power_nc(amps[n]));
+            if ((b & mask) == prefix) atomicAdd(&hist[(b >> shift) & 255u], 1u);
+        }
+        select_digit<T>(hist, sel, k, t);
+        prefix |= (unsigned)sel[0] << shift;
+        mask |= 255u << shift;
+        k = sel[1];
+    }
+    const float med = __uint_as_float(prefix);
+    const float c = quake_rsqrt_s((float)((double)med / kLn2));
+    for (int n = t; n < nhalf; n += T) {
+        cplx a = amps[n];
+        if (n < numdata) a = scale_nc(a, c);
+        out[2 * n] = a;
+        out[2 * n + 1] = mk(0.f, 0.f);
+    }
+    if (t == 0 && med_out) *med_out = med;
+}
+
+int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,
+                    hipStream_t s)
+{
+    hipLaunchKernelGGL(convolve_kernel, dim3((L + 255) / 256, Z), dim3(256), 0, s, d_fft, d_bank,
+                       L, d_out);
+    return check_launch("convolve_kernel");
+}
+
+int launch_powers(const cplx* d_ifft, int Z, int numrs, int L, float scale, float* d_out,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(powers_kernel, dim3((numrs + 255) / 256, Z), dim3(256), 0, s, d_ifft,
+                       numrs, L, scale, d_out);
+    return check_launch("powers_kernel");
+}
+
+int launch_add_powers(int Z0, int numrs0, int Lh, int zlo0, int zlo_h, int rlo_h, double f,
+                      const cplx* d_ifft_h, float scale, float* d_powers, int fullrlo,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(add_powers_kernel, dim3((numrs0 + 255) / 256, Z0), dim3(256), 0, s, Z0,
+                       numrs0, Lh, zlo0, zlo_h, rlo_h, f, d_ifft_h, scale, d_powers, fullrlo);
+    return check_launch("add_powers_kernel");
+}
+
+int launch_normalize_spread(const cplx* d_amps, int numdata, int nhalf, cplx* d_out,
+                            float* d_median_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(normalize_spread_kernel, dim3(1), dim3(1024), 0, s, d_amps, numdata, nhalf,
+                       d_out, d_median_out);
+    return check_launch("normalize_spread_kernel");
+}
+
+}  // namespace fdas

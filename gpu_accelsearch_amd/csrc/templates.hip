@@ -1,0 +1,229 @@
+// templates.hip -- search geometry and the device-resident template bank.
+//
+// Geometry restates source/bpd_subharmonics.cu:10-67 and headers/bpd_math.h:13.
+// Templates (source/bpd_subharmonics.cu:78-279) are generated on the GPU in FP64,
+// one thread per tap, rounded to FP32, placed wrap-around in rows of length L
+// and transformed in place by the row FFT.
+#include <cmath>
+
+#include "fdas_internal.hpp"
+
+namespace fdas {
+
+// ------------------------------------------------------------------ geometry
+int h_nearest_int(double x) { return (int)(x < 0 ? std::ceil(x - 0.5) : std::floor(x + 0.5)); }
+
+int h_calc_required_z(double f, double zfull) { return h_nearest_int(RDZ * zfull * f) * DZ; }
+
+double h_calc_required_r(double f, double rfull) { return (int)(RDR * rfull * f + 0.5) * DR; }
+
+int h_z_response_halfwidth(double z)
+{
+    z = std::fabs(z);
+    int m = (int)(z * (0.00089 * z + 0.3131) + NUMFINTBINS);
+    m = (m < NUMFINTBINS) ? NUMFINTBINS : m;
+    if (z > 100 && m > 0.6 * z) m = (int)(0.6 * z);
+    return m;
+}
+
+long long h_next_pow2(long long x)
+{
+    long long i = 1;
+    while (i < x) i <<= 1;
+    return i;
+}
+
+int h_fast_log2(int n)
+{
+    int x = 0;
+    while (n > 1) {
+        n >>= 1;
+        x++;
+    }
+    return x;
+}
+
+int h_fft_length(int uselen, int numharm, int harmnum, int zmax)
+{
+    const double f = (double)harmnum / (double)numharm;
+    const int bins_needed = (uselen * harmnum) / numharm + 2;
+    const int end_effects = 2 * NUMBETWEEN * h_z_response_halfwidth(h_calc_required_z(f, zmax));
+    return (int)h_next_pow2(bins_needed + end_effects);
+}
+
+int h_max_uselen(int fftlen, int zmax)
+{
+    int u = fftlen & ~1;
+    while (u > 0 && h_fft_length(u, 1, 1, zmax) > fftlen) u -= 2;
+    return u;
+}
+
+void h_harmonic_order(int numharm, int* cur, int* max)
+{
+    const int stages = h_fast_log2(numharm) + 1;
+    int k = 0;
+    for (int s = 0; s < stages; s++) {
+        const int mx = (s < 1) ? 1 : (1 << s);
+        for (int c = 1; c <= mx; c += 2) {
+            cur[k] = c;
+            max[k] = mx;
+            k++;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Fresnel
+// C(x) = int_0^x cos(pi t^2/2) dt, S(x) = int_0^x sin(pi t^2/2) dt in FP64.
+// |x| < 1.5: Maclaurin series  x * sum_k s_k a^k / (k! (2k+1)), a = pi x^2 / 2
+//            (even k -> C, odd k -> S, s_k = + + - - ...).
+// |x| >= 1.5: C + iS = (1+i)/2 (1 - erfc(w)), w = sqrt(pi)/2 (1-i) x, with
+//            erfc(w) = (1-i) x e^{i pi x^2/2} h and h the Laplace continued
+//            fraction 1/(b1 + a2/(b2 + ...)), b_k = 1 - i pi x^2 + 4(k-1),
+//            a_k = -(2k-3)(2k-2), evaluated by the modified Lentz method.
+// |x| > 36974: 0.5 (the saturation of the reference's fresnl, bpd_fresnl.c:170).
+__device__ void fresnel_d(double xin, double* s_out, double* c_out)
+{
+    const double kPi = 3.14159265358979323846;
+    double x = fabs(xin), c, s;
+    if (x < 1.5) {
+        const double a = 0.5 * kPi * x * x;
+        double t = 1.0, sc = 0.0, ss = 0.0;
+        for (int k = 0; k < 60; k++) {
+            if (k > 0) t *= a / k;
+            const double term = t / (2 * k + 1);
+            const double sg = ((k & 3) < 2) ? 1.0 : -1.0;
+            if (k & 1) ss += sg * term;
+            else sc += sg * term;
+            if (t < 1e-18) break;
+        }
+        c = x * sc;
+        s = x * ss;
+    } else if (x > 36974.0) {
+        c = 0.5;
+        s = 0.5;
+    } else {
+        const double px2 = kPi * x * x;
+        double br = 1.0, bi = -px2;
+        double cr = 1e300, ci = 0.0;
+        double den = br * br + bi * bi;
+        double dr = br / den, di = -bi / den;
+        double hr = dr, hi = di;
+        for (int k = 2; k < 2000; k++) {
+            const double n = 2 * k - 3;
+            const double an = -n * (n + 1.0);
+            br += 4.0;
+            const double tr = an * dr + br, ti = an * di + bi;
+            den = tr * tr + ti * ti;
+            dr = tr / den;
+            di = -ti / den;
+            den = cr * cr + ci * ci;
+            cr = br + an * cr / den;
+            ci = bi - an * ci / den;
+            const double er = cr * dr - ci * di, ei = cr * di + ci * dr;
+            const double nr = hr * er - hi * ei, ni = hr * ei + hi * er;
+            hr = nr;
+            hi = ni;
+            if (fabs(er - 1.0) + fabs(ei) < 1e-17) break;
+        }
+        const double gr = (hr + hi) * x, gi = (hi - hr) * x;
+        double sp, cp;
+        sincos(0.5 * px2, &sp, &cp);
+        const double pr = cp * gr - sp * gi, pim = cp * gi + sp * gr;
+        const double ur = 1.0 - pr, ui = -pim;
+        c = 0.5 * (ur - ui);
+        s = 0.5 * (ur + ui);
+    }
+    if (xin < 0.0) {
+        c = -c;
+        s = -s;
+    }
+    *c_out = c;
+    *s_out = s;
+}
+
+// Tap i (of numkern) of the drift-z response with roffset 0 and 2 points per
+// bin: r_i = numkern/4 - z/2 - i/2 and the response is
+// int_0^1 exp(2 pi i (r_i u + z u^2 / 2)) du
+// (source/bpd_subharmonics.cu:146-207; z = 0 uses the interpolation kernel of
+// :78-135, whose centre tap is exactly 1).
+__device__ cplx template_tap(int z, int numkern, int i)
+{
+    const double kPi = 3.14159265358979323846;
+    const double kSqrt2 = 1.41421356237309504880;
+    if (z == 0) {
+        if (i == numkern / 2) return mk(1.0f, 0.0f);
+        const double rr = kPi * (numkern / 4.0) - kPi * 0.5 * i;
+        double sr, cr;
+        sincos(rr, &sr, &cr);
+        const double sinc = sr / rr;
+        return mk((float)(cr * sinc), (float)(sr * sinc));
+    }
+    const double zd_abs = fabs((double)z);
+    const int signz = (z < 0) ? -1 : 1;
+    const double zd = signz * kSqrt2 / sqrt(zd_abs);
+    const double cons = zd / 2.0;
+    const double pibyz = kPi / (double)z;
+    const double r = (0.0 - 0.5 * z) + numkern / 4.0 - 0.5 * i;
+    const double yy = r * zd;
+    const double zz = yy + z * zd;
+    const double xx = pibyz * r * r;
+    double sx, cx;
+    sincos(xx, &sx, &cx);
+    double fsy, fcy, fsz, fcz;
+    fresnel_d(yy, &fsy, &fcy);
+    fresnel_d(zz, &fsz, &fcz);
+    const double tr = signz * (fcz - fcy);
+    const double ti = fsy - fsz;
+    return mk((float)((tr * cx - ti * sx) * cons), (float)(-(tr * sx + ti * cx) * cons));
+}
+
+__device__ __forceinline__ int d_z_response_halfwidth(double z)
+{
+    z = fabs(z);
+    int m = (int)(z * (0.00089 * z + 0.3131) + NUMFINTBINS);
+    m = (m < NUMFINTBINS) ? NUMFINTBINS : m;
+    if (z > 100 && m > 0.6 * z) m = (int)(0.6 * z);
+    return m;
+}
+
+// Row k of the bank (z = -zmax_h + 2k): taps [2hw, 4hw) go to [0, 2hw) and
+// taps [0, 2hw) go to [L-2hw, L) (source/bpd_subharmonics.cu:259-265).  Grid:
+// (tap blocks, rows); the bank is zeroed beforehand.
+__global__ __launch_bounds__(256) void template_rows_kernel(cplx* __restrict__ bank, int zmax_h,
+                                                            int L)
+{
+    const int k = blockIdx.y;
+    const int z = -zmax_h + DZ * k;
+    const int hw = d_z_response_halfwidth((double)z);
+    const int numkern = 2 * NUMBETWEEN * hw;
+    const int half = numkern / 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= numkern) return;
+    const cplx v = template_tap(z, numkern, i);
+    const int pos = (i >= half) ? (i - half) : (L - half + i);
+    bank[(size_t)k * L + pos] = v;
+}
+
+int build_template_bank(cplx* d_bank, int zmax_h, int nz, int L, hipStream_t s)
+{
+    const int hwmax = h_z_response_halfwidth((double)zmax_h);
+    if (4 * hwmax > L) {
+        set_error("template of z=%d (%d taps) does not fit FFT length %d", zmax_h, 4 * hwmax, L);
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_HIP(hipMemsetAsync(d_bank, 0, (size_t)nz * L * sizeof(cplx), s));
+    dim3 grid((unsigned)((4 * hwmax + 255) / 256), (unsigned)nz), block(256);
+    hipLaunchKernelGGL(template_rows_kernel, grid, block, 0, s, d_bank, zmax_h, L);
+    FDAS_TRY(check_launch("template_rows_kernel"));
+    if (L <= 16384) return fft_rows(d_bank, d_bank, L, nz, L, L, -1, 1.0f, 0, s);
+    cplx* work = nullptr;
+    FDAS_HIP(hipMalloc(&work, 2 * (size_t)L * sizeof(cplx)));
+    int rc = FDAS_OK;
+    for (int k = 0; k < nz && rc == FDAS_OK; ++k)
+        rc = fft_c2c(d_bank + (size_t)k * L, d_bank + (size_t)k * L, L, -1, work, s);
+    (void)hipStreamSynchronize(s);
+    (void)hipFree(work);
+    return rc;
+}
+
+}  // namespace fdas

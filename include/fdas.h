@@ -1,0 +1,247 @@
+/*
+ * fdas.h -- C ABI of the MI355X Fourier-domain acceleration search (libfdas.so).
+ *
+ * Two layers:
+ *   1. Reference-named entry points with the reference's signatures, so code
+ *      written against keithasr/gpu_accelsearch links unchanged.  Each names the
+ *      reference declaration it replaces.  Errors never exit(): they are kept in a
+ *      thread-local message (fdas_last_error) and the call returns its "empty"
+ *      value; set FDAS_ABORT_ON_ERROR=1 in the environment to abort instead.
+ *   2. A plan ("fdas_*") API that returns status codes and runs the fused,
+ *      batched search (whole DM trial per call, device-resident data).
+ *
+ * All compute runs in hand-written HIP kernels for gfx950; nothing falls back
+ * to the CPU.  Pointers named d_* / device_* are device memory.
+ */
+#ifndef FDAS_H_
+#define FDAS_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Interleaved single-precision complex; same 8-byte {re, im} layout as the
+ * reference's complex type (headers/bpd_common.h:43 includes it from cufft.h). */
+typedef struct fdas_complex {
+    float x;
+    float y;
+} fdas_complex;
+
+/* headers/bpd_subharmonics.cuh:13-25 (same field order). kernels_data is the
+ * device-resident FFT'd template bank [totalKernelsTemplates][kernelFFTLength]. */
+typedef struct SubharmonicInfo {
+    unsigned short maxHarmonic;
+    unsigned short currentHarmonic;
+    int zmax;
+    unsigned short totalKernelsTemplates;
+    int kernelFFTLength;
+    int *z;
+    unsigned short *kernelHalfWidth;
+    fdas_complex *kernels_data;
+} SubharmonicInfo;
+
+/* headers/bpd_search.cuh:15-32 (same field order). */
+typedef struct ffdotInfo {
+    unsigned short currentHarmonic;
+    unsigned short maxHarmonic;
+    int numrs;
+    int numzs;
+    int rlo;
+    int zlo;
+    double rhi;
+    int fftlen;
+} ffdotInfo;
+
+/* ------------------------------------------------------------------------ */
+/* Search constants. The reference fixes ACCEL_USELEN at compile time        */
+/* (headers/bpd_common.h:24, 32000); here it is a process-wide setting.      */
+/* ------------------------------------------------------------------------ */
+#define FDAS_DEFAULT_USELEN 32000
+int fdas_set_uselen(int uselen);   /* even, < 65536; returns 0 or -1 */
+int fdas_get_uselen(void);
+
+/* ------------------------------------------------------------------------ */
+/* 1. Reference-named entry points                                           */
+/* ------------------------------------------------------------------------ */
+
+/* headers/bpd_subharmonics.cuh:27 -- per-harmonic template banks, generated in
+ * FP64 on the GPU and FFT'd on the GPU; kernels_data stays on the device. */
+SubharmonicInfo *createSubharmonicInfo(int numHarmonicStages, int zhi);
+/* Releases what createSubharmonicInfo allocated (the reference frees the
+ * fields inline, AccelSearch.cu:274-279). */
+void freeSubharmonicInfo(SubharmonicInfo *shi, int totalHarmonics);
+
+/* headers/bpd_subharmonics.cuh:29-33 */
+int calc_required_z(double harmonicFraction, double zfull);
+double calc_required_r(double harmonicFraction, double rfull);
+int index_from_r(double r, double rlo);
+int index_from_z(double z, double zlo);
+int getTotalHarmonics(int harmonicStages);
+/* source/bpd_subharmonics.cu:31 and :60 (file-local there, exported here) */
+int z_response_halfwidth(double z);
+int calculateFFTLength(unsigned short numharm, unsigned short harmnum, int max_zfull);
+
+/* headers/bpd_fft.cuh:14 -- host series in, malloc'd host spectrum out
+ * (n/2+1 bins; caller free()s). Returns the bin count, 0 on error. */
+unsigned realToComplexFFT(float *idata, long inputElements, fdas_complex **odata, int batch);
+/* headers/bpd_fft.cuh:15-16 -- batched forward / inverse C2C on device data;
+ * *device_odata is allocated here (caller releases it with fdas_free). */
+unsigned complexToComplexFFT(fdas_complex *device_idata, unsigned int inputElements,
+                             fdas_complex **device_odata, int batch);
+unsigned inverseComplexToComplexFFT(fdas_complex *device_idata, unsigned int inputElements,
+                                    fdas_complex **device_odata, int batch);
+
+/* headers/bpd_noise.h:16 -- whitening of host bins [0, numOfBins), in place. */
+void deredden(fdas_complex *fft, unsigned numOfBins);
+
+/* headers/bpd_search.cuh:34-35 -- host data movement helpers. */
+fdas_complex *get_fourier_amplitudes(int loBin, int numbins, fdas_complex *fft,
+                                     long dataNumbins, long dataLoBin);
+void spreadData(fdas_complex *inputData, int inputElements, int interval,
+                unsigned int startIdx, fdas_complex *outputData);
+
+/* headers/bpd_search.cuh:37-46 */
+void generate_harmonics_ffdot(unsigned short totalHarmonics, unsigned short harmonicStages,
+                              SubharmonicInfo *subharmonicsInfo, double fullrlo, double fullrhi,
+                              double zlo, ffdotInfo *output_ffdot,
+                              unsigned short *output_binOffsetArray);
+void generate_harmonic_ffdot_data(unsigned short totalHarmonics, long totalDataLength,
+                                  ffdotInfo *ffdot, unsigned short *binoffsetArray,
+                                  long *dataStartIdx, fdas_complex *fft, long dataNumbins,
+                                  long dataLoBin, fdas_complex *output_data);
+void perform_fft_harmonics(unsigned short totalHarmonics, fdas_complex *device_data,
+                           long *dataStartIdx, fdas_complex *output_device_fftdata);
+void perform_convolution(unsigned short totalHarmonics, fdas_complex *device_fft_data,
+                         ffdotInfo *harmonicInfo, SubharmonicInfo *shi, long *dataStartIdx,
+                         fdas_complex *output_deviceConvolvedSignal,
+                         long *convolutionHarmonicIdx);
+void perform_inversefft(unsigned short totalHarmonics, fdas_complex *device_convolvedSignal,
+                        long *convolutionHarmonicIdx, fdas_complex *output_device_ifftdata);
+void generate_fundamental_powers(unsigned short totalHarmonics, fdas_complex *device_ifft_data,
+                                 long *ifftHarmonicIdx, ffdotInfo *harmonicInfo,
+                                 SubharmonicInfo *shi, float *output_devicePowers);
+void perform_harmonic_summing(unsigned short totalHarmonics, fdas_complex *device_ifft_data,
+                              long *ifftHarmonicIdx, ffdotInfo *harmonicInfo,
+                              SubharmonicInfo *shi, float *output_devicePowers, int fullrlo,
+                              unsigned short *host_rinds, unsigned short *device_rinds,
+                              int totalRindsBins);
+void initialize_cuda_plans(unsigned short totalHarmonics, SubharmonicInfo *subharmonicInfo);
+void free_cuda_plans(unsigned short totalHarmonics);
+
+/* ------------------------------------------------------------------------ */
+/* 2. Plan API                                                               */
+/* ------------------------------------------------------------------------ */
+enum {
+    FDAS_OK = 0,
+    FDAS_ERR_INVALID = 1,     /* bad parameter                               */
+    FDAS_ERR_HIP = 2,         /* HIP runtime error                           */
+    FDAS_ERR_NOMEM = 3,       /* device allocation failed                    */
+    FDAS_ERR_UNSUPPORTED = 4  /* shape outside what the kernels implement    */
+};
+
+enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2 };
+
+#define FDAS_MAX_HARM 16
+#define FDAS_MAX_STAGES 5
+
+typedef struct fdas_params {
+    long n;             /* time-series length (even)                          */
+    float dt;           /* sample time, s; rlo = floor(n*dt) as in the driver */
+    int zmax;           /* 0..1200, even                                      */
+    int numharm;        /* 1, 2, 4, 8 or 16                                   */
+    int fftlen;         /* fundamental FFT length, 1024..16384                */
+    int uselen;         /* 0: largest even U that fits fftlen                 */
+    int precision;      /* FDAS_PREC_*                                        */
+    int chunk_steps;    /* steps per batch of launches, 0: automatic          */
+    int max_candidates; /* candidate buffer capacity, 0: 1<<20                */
+    float powcut[FDAS_MAX_STAGES]; /* per stage (1,2,4,8,16 harmonics);
+                                      <= 0 disables that stage's threshold  */
+} fdas_params;
+
+typedef struct fdas_plan_info {
+    int uselen;
+    int rlo;
+    int nsteps;
+    int numharm;
+    int numz;                          /* fundamental templates (rows)   */
+    int fftlen[FDAS_MAX_HARM];
+    int numzs[FDAS_MAX_HARM];
+    int harm_cur[FDAS_MAX_HARM];
+    int harm_max[FDAS_MAX_HARM];
+    int chunk_steps;
+    long nbins;                        /* n/2: bins whitened and searched */
+    long long cells;                   /* nsteps * uselen * numz          */
+    long long sub_cells[FDAS_MAX_HARM]; /* sum over steps of numzs*numrs   */
+    size_t device_bytes;
+} fdas_plan_info;
+
+typedef struct fdas_candidate {
+    int step;      /* step index                                         */
+    int z_index;   /* fundamental row, z = -zmax + 2*z_index             */
+    int r_index;   /* column, r = rlo + step*uselen/2 + 0.5*r_index      */
+    int stage;     /* 0..4: summed over 1,2,4,8,16 harmonics             */
+    float power;   /* summed normalised power                            */
+    float reserved;
+} fdas_candidate;
+
+typedef struct fdas_plan fdas_plan;
+
+int fdas_plan_create(const fdas_params *params, fdas_plan **plan);
+void fdas_plan_destroy(fdas_plan *plan);
+int fdas_plan_get_info(const fdas_plan *plan, fdas_plan_info *info);
+/* Stream the plan's work is queued on (a hipStream_t; NULL = the plan's own). */
+int fdas_plan_set_stream(fdas_plan *plan, void *hip_stream);
+
+/* R2C FFT + whitening of a device series of params->n floats; the whitened
+ * spectrum stays in the plan.  Resets the candidate list. */
+int fdas_prepare(fdas_plan *plan, const float *d_series);
+/* Steps [step0, step0+nsteps) of the prepared spectrum.  If d_planes is not
+ * NULL the summed f-fdot plane of step s is written to
+ * d_planes + (s-step0)*numz*uselen (row z, column i).  Candidates append. */
+int fdas_run_steps(fdas_plan *plan, int step0, int nsteps, float *d_planes);
+/* prepare + all steps (one DM trial). */
+int fdas_search(fdas_plan *plan, const float *d_series);
+/* Copy of the whitened spectrum (nbins complex) to device memory d_out. */
+int fdas_get_spectrum(fdas_plan *plan, fdas_complex *d_out);
+/* Candidates found since fdas_prepare, in (step, z, r, stage) order. */
+int fdas_get_candidates(fdas_plan *plan, fdas_candidate *out, int capacity, int *count,
+                        int *overflowed);
+int fdas_synchronize(fdas_plan *plan);
+
+/* Per-kernel timing with HIP events recorded on the plan's stream around each
+ * launch (enable resets the counters; adds two event records per launch). */
+enum {
+    FDAS_K_R2C = 0,        /* real-to-complex transform of the series          */
+    FDAS_K_WHITEN = 1,     /* block medians + normalisation                    */
+    FDAS_K_SEGPREP = 2,    /* segment fetch, median, normalise, forward FFT    */
+    FDAS_K_CORR_SUB = 3,   /* fused correlation/IFFT/power, sub-harmonics      */
+    FDAS_K_CORR_FUND = 4,  /* fused correlation/IFFT/power + harmonic sum      */
+    FDAS_K_NCAT = 5
+};
+typedef struct fdas_timing {
+    int launches[8];
+    double ms[8];
+} fdas_timing;
+int fdas_plan_enable_timing(fdas_plan *plan, int enable);
+int fdas_plan_get_timing(fdas_plan *plan, fdas_timing *out);
+
+/* ------------------------------------------------------------------------ */
+/* Memory plumbing and diagnostics                                           */
+/* ------------------------------------------------------------------------ */
+int fdas_malloc(void **d_ptr, size_t bytes);
+int fdas_free(void *d_ptr);
+int fdas_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
+int fdas_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
+int fdas_memset(void *d_dst, int value, size_t bytes);
+int fdas_device_synchronize(void);
+const char *fdas_last_error(void);
+void fdas_clear_error(void);
+const char *fdas_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDAS_H_ */

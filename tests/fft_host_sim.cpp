@@ -1,0 +1,64 @@
+// Host simulation of the workgroup FFT in gpu_accelsearch_amd/csrc/fft_block.hpp.
+// Each phase (compute / LDS store / LDS load) runs for every simulated thread in
+// turn, which is what the barriers between phases guarantee on the GPU, so the
+// Stockham index arithmetic and the twiddle tables are checked without a GPU.
+// Built by tests/test_fft_host_sim.py with g++ and called through ctypes.
+#include "../gpu_accelsearch_amd/csrc/fft_block.hpp"
+
+#include <vector>
+
+using namespace fdas;
+
+template <int N, int T, int DIR, int P>
+static void run_rest(std::vector<cplx>& v, std::vector<cplx>& lds, const std::vector<cplx>& tw)
+{
+    using F = BlockFFT<N, T, DIR>;
+    if constexpr (P < F::NPASS) {
+        for (int t = 0; t < T; ++t) F::template store<P - 1>(&v[(size_t)t * F::E], lds.data(), t);
+        for (int t = 0; t < T; ++t) F::template load<P>(&v[(size_t)t * F::E], lds.data(), t);
+        for (int t = 0; t < T; ++t) F::template compute<P>(&v[(size_t)t * F::E], t, tw.data());
+        run_rest<N, T, DIR, P + 1>(v, lds, tw);
+    }
+}
+
+template <int N, int T, int DIR>
+static void sim(const float* in, float* out)
+{
+    using F = BlockFFT<N, T, DIR>;
+    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds(F::LDS_ELEMS);
+    F::build_twiddles(tw.data());
+    std::vector<cplx> v((size_t)T * F::E);
+    for (int t = 0; t < T; ++t)
+        for (int i = 0; i < F::E; ++i) {
+            int n = F::in_index(t, i);
+            v[(size_t)t * F::E + i] = mk(in[2 * n], in[2 * n + 1]);
+        }
+    for (int t = 0; t < T; ++t) F::template compute<0>(&v[(size_t)t * F::E], t, tw.data());
+    run_rest<N, T, DIR, 1>(v, lds, tw);
+    for (int t = 0; t < T; ++t)
+        for (int i = 0; i < F::E; ++i) {
+            int n = F::out_index(t, i);
+            out[2 * n] = v[(size_t)t * F::E + i].x;
+            out[2 * n + 1] = v[(size_t)t * F::E + i].y;
+        }
+}
+
+// The (N, T) shapes the library instantiates.
+extern "C" int fft_sim(int n, int dir, const float* in, float* out)
+{
+#define CASE(NN, TT)                                          \
+    if (n == NN) {                                            \
+        if (dir < 0) sim<NN, TT, -1>(in, out);                \
+        else sim<NN, TT, +1>(in, out);                        \
+        return 0;                                             \
+    }
+    CASE(256, 64)
+    CASE(512, 64)
+    CASE(1024, 64)
+    CASE(2048, 128)
+    CASE(4096, 256)
+    CASE(8192, 256)
+    CASE(16384, 512)
+#undef CASE
+    return -1;
+}

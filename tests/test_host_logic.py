@@ -1,0 +1,137 @@
+"""Host-side logic of the product that needs no GPU: the C ABI loads and exports
+every entry point declared in include/fdas.h, parameter validation rejects bad
+plans, host data movement helpers, and the workgroup FFT's index arithmetic
+(tests/fft_host_sim.cpp simulates a workgroup phase by phase)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "fdas.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?[\s\*](\w+)\s*\(", src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "sizeof")))
+
+
+def test_abi_exports_every_declared_symbol():
+    import gpu_accelsearch_amd as fd
+    L = fd.lib()
+    names = declared_functions()
+    assert len(names) > 40
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    from gpu_accelsearch_amd._native import PROTOTYPES
+    bound = {p[0] for p in PROTOTYPES}
+    assert set(names) <= bound, set(names) - bound
+
+
+@pytest.mark.parametrize("kw,code", [
+    (dict(n=1000), 4), (dict(zmax=201), 1), (dict(zmax=1300), 1), (dict(numharm=3), 1),
+    (dict(fftlen=512), 4), (dict(fftlen=6000), 4), (dict(uselen=7735), 1),
+    (dict(uselen=8000), 1), (dict(precision="bf16", fftlen=8192), 4),
+])
+def test_plan_rejects_bad_parameters(kw, code):
+    import gpu_accelsearch_amd as fd
+    base = dict(n=1 << 20, zmax=200, numharm=8, fftlen=8192)
+    base.update(kw)
+    p = fd.SearchParams(**base).to_c()
+    h = C.c_void_p()
+    rc = fd.lib().fdas_plan_create(C.byref(p), C.byref(h))
+    assert rc == code, (kw, rc, fd.last_error())
+    assert h.value is None
+    assert fd.last_error()
+
+
+def test_uselen_setting():
+    import gpu_accelsearch_amd as fd
+    L = fd.lib()
+    assert L.fdas_get_uselen() == 32000
+    assert L.fdas_set_uselen(7735) == -1
+    assert L.fdas_set_uselen(7734) == 0
+    assert fd.calculateFFTLength(1, 1, 200) == 8192
+    assert L.fdas_set_uselen(32000) == 0
+    assert fd.calculateFFTLength(1, 1, 300) == 32768
+
+
+def test_host_data_helpers():
+    """get_fourier_amplitudes / spreadData (source/bpd_search.cu:16-67)."""
+    import gpu_accelsearch_amd as fd
+    L = fd.lib()
+    spec = (np.arange(20) + 1j * np.arange(20)).astype(np.complex64)
+    p = L.get_fourier_amplitudes(-3, 8, spec.ctypes.data, 20, 0)
+    got = np.frombuffer((C.c_float * 16).from_address(p), np.float32).copy().view(np.complex64)
+    C.CDLL(None).free(C.c_void_p(p))
+    np.testing.assert_array_equal(got, np.concatenate([np.zeros(3, np.complex64), spec[:5]]))
+    p = L.get_fourier_amplitudes(16, 8, spec.ctypes.data, 20, 0)
+    got = np.frombuffer((C.c_float * 16).from_address(p), np.float32).copy().view(np.complex64)
+    C.CDLL(None).free(C.c_void_p(p))
+    np.testing.assert_array_equal(got, np.concatenate([spec[16:], np.zeros(4, np.complex64)]))
+    out = np.full(20, 7 + 7j, np.complex64)
+    L.spreadData(spec.ctypes.data, 5, 2, 3, out.ctypes.data)
+    want = np.full(20, 7 + 7j, np.complex64)
+    want[3:13] = 0
+    want[3:13:2] = spec[:5]
+    np.testing.assert_array_equal(out, want)
+
+
+def test_generate_harmonics_ffdot_matches_oracle(oracle):
+    """Per-step ffdotInfo from the exported host function vs the oracle (needs a
+    SubharmonicInfo only for numzs/fftlen/halfwidth; built here by hand)."""
+    import gpu_accelsearch_amd as fd
+    from gpu_accelsearch_amd._native import FfdotInfo, SubharmonicInfo
+    L = fd.lib()
+    L.fdas_set_uselen(7734)
+    try:
+        H, stages, zmax = 8, 4, 200
+        order = oracle.harmonic_order(H)
+        shi = (SubharmonicInfo * H)()
+        keep = []
+        for h, (c, m) in enumerate(order):
+            zh = fd.calc_required_z(c / m, zmax)
+            shi[h].maxHarmonic, shi[h].currentHarmonic, shi[h].zmax = m, c, zh
+            shi[h].totalKernelsTemplates = zh + 1
+            shi[h].kernelFFTLength = fd.calculateFFTLength(m, c, zmax)
+            hw = (C.c_ushort * (zh + 1))(*[fd.z_response_halfwidth(-zh + 2 * k) for k in range(zh + 1)])
+            keep.append(hw)
+            shi[h].kernelHalfWidth = C.cast(hw, C.POINTER(C.c_ushort))
+        for startr in (536.0, 536.0 + 3867 * 500, 536.0 + 3867 * 1083):
+            info = (FfdotInfo * H)()
+            bo = (C.c_ushort * H)()
+            L.generate_harmonics_ffdot(H, stages, shi, startr, startr + 3867 - 0.5, -zmax, info, bo)
+            want = oracle.step_geometry(H, zmax, 7734, startr)
+            for h in range(H):
+                w = want[h]
+                assert (info[h].currentHarmonic, info[h].maxHarmonic) == (w["cur"], w["max"])
+                assert (info[h].numrs, info[h].numzs, info[h].rlo, info[h].zlo, info[h].fftlen) == \
+                    (w["numrs"], w["numzs"], w["rlo"], w["zlo"], w["fftlen"])
+                assert info[h].rhi == w["rhi"] and bo[h] == w["binoffset"]
+    finally:
+        L.fdas_set_uselen(32000)
+
+
+@pytest.fixture(scope="module")
+def fft_sim(tmp_path_factory):
+    d = tmp_path_factory.mktemp("fftsim")
+    so = str(d / "fft_sim.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-o", so,
+                    os.path.join(ROOT, "tests", "fft_host_sim.cpp")], check=True)
+    return C.CDLL(so)
+
+
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096, 8192, 16384])
+def test_workgroup_fft_index_arithmetic(fft_sim, n):
+    rng = np.random.default_rng(n)
+    for sign in (-1, 1):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+        out = np.zeros(n, np.complex64)
+        rc = fft_sim.fft_sim(n, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
+        assert rc == 0
+        ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
+        assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
