@@ -179,10 +179,13 @@ extern "C" void freeSubharmonicInfo(SubharmonicInfo* shi, int totalHarmonics)
 // ====================================================================== FFTs
 extern "C" unsigned realToComplexFFT(float* idata, long n, fdas_complex** odata, int batch)
 {
-    if (!idata || !odata || n < 128 || batch < 1 || (n & (n - 1))) {
-        set_error("realToComplexFFT: length %ld (power of two >= 128 required)", n);
+    if (!idata || !odata || batch < 1) {
+        set_error("realToComplexFFT: null argument");
         return 0;
     }
+    *odata = nullptr;
+    R2CPlan rp;
+    if (r2c_plan_create(n, &rp, 0) != FDAS_OK) return 0;
     const long nout = n / 2 + 1;
     float* d_in = nullptr;
     cplx *d_out = nullptr, *work = nullptr;
@@ -190,12 +193,12 @@ extern "C" unsigned realToComplexFFT(float* idata, long n, fdas_complex** odata,
     fdas_complex* host = (fdas_complex*)malloc(sizeof(fdas_complex) * nout * batch);
     if (hipMalloc(&d_in, sizeof(float) * n) == hipSuccess &&
         hipMalloc(&d_out, sizeof(cplx) * nout) == hipSuccess &&
-        hipMalloc(&work, sizeof(cplx) * fft_r2c_work_elems(n)) == hipSuccess) {
+        hipMalloc(&work, sizeof(cplx) * rp.work_elems) == hipSuccess) {
         ret = (unsigned)nout;
         for (int b = 0; b < batch && ret; ++b) {
             if (hipMemcpy(d_in, idata + (size_t)b * n, sizeof(float) * n, hipMemcpyHostToDevice) !=
                     hipSuccess ||
-                fft_r2c(d_in, d_out, n, work, 0) != FDAS_OK ||
+                fft_r2c_plan(rp, d_in, d_out, work, 0) != FDAS_OK ||
                 hipMemcpy(host + (size_t)b * nout, d_out, sizeof(cplx) * nout,
                           hipMemcpyDeviceToHost) != hipSuccess)
                 ret = 0;
@@ -206,6 +209,7 @@ extern "C" unsigned realToComplexFFT(float* idata, long n, fdas_complex** odata,
     (void)hipFree(d_in);
     (void)hipFree(d_out);
     (void)hipFree(work);
+    r2c_plan_destroy(&rp);
     if (!ret) {
         free(host);
         host = nullptr;
@@ -495,6 +499,7 @@ struct fdas_plan {
     int* d_rlo_h = nullptr;      // [nsteps][MAX_HARM]
     cplx* d_spec = nullptr;
     cplx* d_r2c_work = nullptr;
+    R2CPlan r2c;
     WhitenSchedule ws;
     cplx* d_seg = nullptr;
     float* d_sub = nullptr;
@@ -548,8 +553,8 @@ static int plan_alloc(fdas_plan* pl, void** ptr, size_t bytes)
 static int plan_build(fdas_plan* pl)
 {
     const fdas_params& p = pl->p;
-    if (p.n < 256 || (p.n & (p.n - 1))) {
-        set_error("series length %ld: power of two >= 256 required", p.n);
+    if (p.n < 256 || (p.n & 1) || p.n > (1L << 27)) {
+        set_error("series length %ld: even and in [256, 2^27] required", p.n);
         return FDAS_ERR_UNSUPPORTED;
     }
     if (p.zmax < 0 || p.zmax > 1200 || (p.zmax & 1)) {
@@ -606,8 +611,8 @@ static int plan_build(fdas_plan* pl)
         pl->zlo[h] = h_calc_required_z(f, -p.zmax);
         pl->seg_off[h] = so;
         so += pl->L[h] / 2;
-        if (pl->L[h] < 1024 || pl->L[h] > 16384) {
-            set_error("harmonic %d/%d needs FFT length %d (fused kernels: 1024..16384)",
+        if (pl->L[h] < 256 || pl->L[h] > 16384) {
+            set_error("harmonic %d/%d needs FFT length %d (fused kernels: 256..16384)",
                       pl->cur[h], pl->mx[h], pl->L[h]);
             return FDAS_ERR_UNSUPPORTED;
         }
@@ -729,7 +734,8 @@ static int plan_build(fdas_plan* pl)
     FDAS_HIP(hipMemcpy(pl->d_fullrlo, fullrlo.data(), (size_t)S * sizeof(int), hipMemcpyHostToDevice));
     FDAS_HIP(hipMemcpy(pl->d_rlo_h, rlo_h.data(), rlo_h.size() * sizeof(int), hipMemcpyHostToDevice));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_spec, (size_t)(p.n / 2 + 1) * sizeof(cplx)));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_r2c_work, fft_r2c_work_elems(p.n) * sizeof(cplx)));
+    FDAS_TRY(r2c_plan_create(p.n, &pl->r2c, pl->stream));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_r2c_work, pl->r2c.work_elems * sizeof(cplx)));
     FDAS_TRY(whiten_schedule_create((unsigned)pl->nbins, &pl->ws));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg, (size_t)pl->chunk * pl->seg_stride * sizeof(cplx)));
     if (sub > 0)
@@ -774,6 +780,7 @@ extern "C" void fdas_plan_destroy(fdas_plan* pl)
     (void)hipFree(pl->d_rlo_h);
     (void)hipFree(pl->d_spec);
     (void)hipFree(pl->d_r2c_work);
+    r2c_plan_destroy(&pl->r2c);
     if (pl->ws.d_start) whiten_schedule_destroy(&pl->ws);
     (void)hipFree(pl->d_seg);
     (void)hipFree(pl->d_sub);
@@ -860,7 +867,7 @@ extern "C" int fdas_prepare(fdas_plan* pl, const float* d_series)
     }
     {
         TimedLaunch tl(pl, FDAS_K_R2C);
-        FDAS_TRY(fft_r2c(d_series, pl->d_spec, pl->p.n, pl->d_r2c_work, pl->stream));
+        FDAS_TRY(fft_r2c_plan(pl->r2c, d_series, pl->d_spec, pl->d_r2c_work, pl->stream));
     }
     {
         TimedLaunch tl(pl, FDAS_K_WHITEN);

@@ -46,10 +46,18 @@ int transpose(const cplx* in, cplx* out, long rows, long cols, hipStream_t s);
 // Complex transform of any power-of-two length 64..2^28 (four-step beyond 16384).
 // work: 2*m complex scratch (unused for m <= 16384).  out may equal in.
 int fft_c2c(const cplx* in, cplx* out, long m, int dir, cplx* work, hipStream_t s);
-// Real-to-complex transform of n (power of two) floats -> n/2+1 bins.
-// work: 3*(n/2) complex scratch.
-int fft_r2c(const float* in, cplx* out, long n, cplx* work, hipStream_t s);
-size_t fft_r2c_work_elems(long n);
+// Real-to-complex transform of n (even) floats -> n/2+1 bins.  The n/2-point
+// complex transform of the (even, odd) pairs is a four-step FFT when n/2 is a
+// power of two, else a chirp-z (Bluestein) transform through length p >= n-1.
+struct R2CPlan {
+    long n = 0, m = 0, p = 0;
+    bool pow2 = true;
+    size_t work_elems = 0;     // complex scratch the transform needs
+    cplx* d_kernel = nullptr;  // FFT of the chirp (Bluestein only)
+};
+int r2c_plan_create(long n, R2CPlan* rp, hipStream_t s);
+void r2c_plan_destroy(R2CPlan* rp);
+int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipStream_t s);
 
 // ---------------------------------------------------------------- whitening
 struct WhitenSchedule {

@@ -240,24 +240,125 @@ int fft_c2c(const cplx* in, cplx* out, long m, int dir, cplx* work, hipStream_t 
     return FDAS_OK;
 }
 
-size_t fft_r2c_work_elems(long n) { return (size_t)3 * (size_t)(n / 2); }
-
-int fft_r2c(const float* in, cplx* out, long n, cplx* work, hipStream_t s)
+// ------------------------------------------------------------------ Bluestein
+// m-point forward DFT of any length through power-of-two transforms of length
+// p >= 2m-1:  Z[k] = conj(c_k) * sum_j (z_j conj(c_j)) c_{k-j},  c_j = exp(i pi j^2/m).
+// The chirp phase uses j^2 mod 2m in integers, so it is exact for any m.
+__device__ __forceinline__ cplx chirp(long long j, long long m)
 {
-    if (n < 128 || !is_pow2(n)) {
-        set_error("fft_r2c: length %ld is not a power of two >= 128", n);
+    const long long q = (j * j) % (2 * m);
+    double sn, cs;
+    sincospi((double)q / (double)m, &sn, &cs);
+    return mk((float)cs, (float)sn);
+}
+
+// a[j] = z[j] conj(c_j) for j < m, 0 for m <= j < p
+__global__ __launch_bounds__(256) void bluestein_pre_kernel(const cplx* __restrict__ z,
+                                                            cplx* __restrict__ a, long m, long p)
+{
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p) return;
+    a[j] = j < m ? cmul(z[j], conjg(chirp(j, m))) : mk(0.f, 0.f);
+}
+
+// b[j] = c_j for j < m, b[p-j] = c_j for 0 < j < m, zero elsewhere
+__global__ __launch_bounds__(256) void bluestein_kernel_kernel(cplx* __restrict__ b, long m, long p)
+{
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p) return;
+    cplx v = mk(0.f, 0.f);
+    if (j < m) v = chirp(j, m);
+    else if (j > p - m) v = chirp(p - j, m);
+    b[j] = v;
+}
+
+__global__ __launch_bounds__(256) void pointwise_mul_kernel(cplx* __restrict__ a,
+                                                            const cplx* __restrict__ b, long p,
+                                                            float scale)
+{
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p) return;
+    const cplx v = cmul(a[j], b[j]);
+    a[j] = mk(v.x * scale, v.y * scale);
+}
+
+__global__ __launch_bounds__(256) void bluestein_post_kernel(const cplx* __restrict__ y,
+                                                             cplx* __restrict__ Z, long m)
+{
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    Z[k] = cmul(y[k], conjg(chirp(k, m)));
+}
+
+static unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
+
+int r2c_plan_create(long n, R2CPlan* rp, hipStream_t s)
+{
+    *rp = R2CPlan();
+    if (n < 256 || (n & 1)) {
+        set_error("real transform length %ld must be even and >= 256", n);
         return FDAS_ERR_UNSUPPORTED;
     }
+    rp->n = n;
+    rp->m = n / 2;
+    rp->pow2 = is_pow2(rp->m);
+    if (rp->pow2) {
+        rp->p = rp->m;
+        rp->work_elems = (size_t)3 * rp->m;
+        return FDAS_OK;
+    }
+    rp->p = 1;
+    while (rp->p < 2 * rp->m - 1) rp->p <<= 1;
+    if (rp->p > (1L << 28)) {
+        set_error("real transform length %ld too long for the chirp-z path", n);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    // Z (m) + a (p) + four-step scratch (2p)
+    rp->work_elems = (size_t)rp->m + 3 * (size_t)rp->p;
+    FDAS_HIP(hipMalloc(&rp->d_kernel, (size_t)rp->p * sizeof(cplx)));
+    cplx* scratch = nullptr;
+    FDAS_HIP(hipMalloc(&scratch, 2 * (size_t)rp->p * sizeof(cplx)));
+    hipLaunchKernelGGL(bluestein_kernel_kernel, dim3(blocks_for(rp->p)), dim3(256), 0, s,
+                       rp->d_kernel, rp->m, rp->p);
+    int rc = check_launch("bluestein_kernel_kernel");
+    if (rc == FDAS_OK) rc = fft_c2c(rp->d_kernel, rp->d_kernel, rp->p, -1, scratch, s);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == FDAS_OK) rc = FDAS_ERR_HIP;
+    (void)hipFree(scratch);
+    return rc;
+}
+
+void r2c_plan_destroy(R2CPlan* rp)
+{
+    (void)hipFree(rp->d_kernel);
+    *rp = R2CPlan();
+}
+
+int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipStream_t s)
+{
     if (((uintptr_t)in) & 7) {
-        set_error("fft_r2c: input must be 8-byte aligned");
+        set_error("real transform input must be 8-byte aligned");
         return FDAS_ERR_INVALID;
     }
-    const long m = n / 2;
-    cplx* Z = work;              // m
-    cplx* scratch = work + m;    // 2m
-    FDAS_TRY(fft_c2c(reinterpret_cast<const cplx*>(in), Z, m, -1, scratch, s));
-    dim3 grid((unsigned)((m + 1 + 255) / 256)), block(256);
-    hipLaunchKernelGGL(r2c_split_kernel, grid, block, 0, s, Z, out, m);
+    const long m = rp.m;
+    cplx* Z = work;
+    const cplx* z = reinterpret_cast<const cplx*>(in);
+    if (rp.pow2) {
+        FDAS_TRY(fft_c2c(z, Z, m, -1, work + m, s));
+    } else {
+        const long p = rp.p;
+        cplx* a = work + m;
+        cplx* scratch = a + p;
+        hipLaunchKernelGGL(bluestein_pre_kernel, dim3(blocks_for(p)), dim3(256), 0, s, z, a, m, p);
+        FDAS_TRY(check_launch("bluestein_pre_kernel"));
+        FDAS_TRY(fft_c2c(a, a, p, -1, scratch, s));
+        hipLaunchKernelGGL(pointwise_mul_kernel, dim3(blocks_for(p)), dim3(256), 0, s, a,
+                           rp.d_kernel, p, (float)(1.0 / (double)p));
+        FDAS_TRY(check_launch("pointwise_mul_kernel"));
+        FDAS_TRY(fft_c2c(a, a, p, +1, scratch, s));
+        hipLaunchKernelGGL(bluestein_post_kernel, dim3(blocks_for(m)), dim3(256), 0, s, a, Z, m);
+        FDAS_TRY(check_launch("bluestein_post_kernel"));
+    }
+    hipLaunchKernelGGL(r2c_split_kernel, dim3(blocks_for(m + 1)), dim3(256), 0, s, Z, out, m);
     return check_launch("r2c_split_kernel");
 }
 

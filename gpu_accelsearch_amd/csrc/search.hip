@@ -18,6 +18,7 @@
 // powers and add_powers kernels plus the normalise-and-spread kernel used by the
 // reference-named entry points.
 #include <cmath>
+#include <cstdlib>
 
 #include "fdas_internal.hpp"
 #include "fft_block.hpp"
@@ -26,6 +27,8 @@ namespace fdas {
 
 template <int N>
 struct FftCfgS;
+template <> struct FftCfgS<128>   { static constexpr int T = 64; };
+template <> struct FftCfgS<256>   { static constexpr int T = 64; };
 template <> struct FftCfgS<512>   { static constexpr int T = 64; };
 template <> struct FftCfgS<1024>  { static constexpr int T = 64; };
 template <> struct FftCfgS<2048>  { static constexpr int T = 128; };
@@ -149,100 +152,198 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     for (int i = 0; i < F::E; ++i) o[F::out_index(t, i)] = v[i];
 }
 
-// Shared body: y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.
+// Template values of row z at this thread's first-pass positions, kept in
+// registers while the workgroup walks its group of steps.
 template <int L>
-__device__ __forceinline__ void corr_ifft(const cplx* __restrict__ K, const cplx* __restrict__ A,
+__device__ __forceinline__ void load_template(const cplx* __restrict__ K, cplx* kr, int t)
+{
+    constexpr int T = FftCfgS<L>::T;
+    using F = BlockFFT<L, T, 1>;
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) kr[i] = K[F::in_index(t, i)];
+}
+
+// y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.  Position t + i*T needs
+// A[t + (i mod E/2)*T], so each thread loads E/2 segment values.
+template <int L>
+__device__ __forceinline__ void corr_ifft(const cplx* kr, const cplx* __restrict__ A,
                                           const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
 {
     constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
+    constexpr int EH = F::E / 2;
+    cplx a[EH];
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) {
-        const int n = F::in_index(t, i);
-        v[i] = cmulc(A[n & (L / 2 - 1)], K[n]);
-    }
+    for (int i = 0; i < EH; ++i) a[i] = A[t + i * T];
+#pragma unroll
+    for (int i = 0; i < F::E; ++i) v[i] = cmulc(a[i % EH], kr[i]);
     F::run(v, lds, tw, t);
 }
 
+// Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps)).
 template <int L>
 __global__ __launch_bounds__(FftCfgS<L>::T) void corr_sub_kernel(
     const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    long long seg_off, int off, int W, float* __restrict__ out, const cplx* __restrict__ tw,
-    float scale)
+    long long seg_off, int nsteps, int G, int off, int W, float* __restrict__ out,
+    const cplx* __restrict__ tw, float scale)
 {
     constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
     __shared__ cplx lds[F::LDS_ELEMS];
-    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-    cplx v[F::E];
-    corr_ifft<L>(bank + (size_t)z * L, seg + s * seg_stride + seg_off, tw, lds, v, t);
-    float* o = out + ((size_t)s * Z + z) * W;
+    const int z = blockIdx.x, t = threadIdx.x;
+    const int s0 = blockIdx.y * G, s1 = min(s0 + G, nsteps);
+    cplx kr[F::E], v[F::E];
+    load_template<L>(bank + (size_t)z * L, kr, t);
+    for (int s = s0; s < s1; ++s) {
+        corr_ifft<L>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+        float* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) {
-        const int col = F::out_index(t, i) - off;
-        if (col >= 0 && col < W) o[col] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+        for (int i = 0; i < F::E; ++i) {
+            const int col = F::out_index(t, i) - off;
+            if (col >= 0 && col < W) o[col] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+        }
     }
 }
 
+__device__ __forceinline__ void emit_candidate(fdas_candidate* cands, int* ncand, int cap,
+                                               int step, int z, int col, int stage, float p)
+{
+    const int idx = atomicAdd(ncand, 1);
+    if (idx < cap) {
+        fdas_candidate c;
+        c.step = step;
+        c.z_index = z;
+        c.r_index = col;
+        c.stage = stage;
+        c.power = p;
+        c.reserved = 0.f;
+        cands[idx] = c;
+    }
+}
+
+// Sub-harmonic column read by fundamental column col (source/bpd_gpu_kernels.cu:139-141):
+// subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm and rr = fullrlo + col/2.
+__device__ __forceinline__ int sub_rind(long long q, int c, int lm, int rlo2)
+{
+    return (int)(((q * c + (1LL << (lm - 1))) >> lm) - rlo2);
+}
+
+// Fundamental: y = IFFT(A * conj K), power, then the harmonic sum in the reference
+// order.  The sub-harmonic row segments a workgroup needs (row zind_h, columns
+// rind(0)..rind(U-1), about U*f_h floats each) are staged in LDS with coalesced
+// loads, as many harmonics at a time as the FFT's LDS buffer holds, and gathered
+// from there.  Stage thresholds set per-stage bit masks; the rare candidates are
+// emitted afterwards (their partial sums recomputed from the sub planes).
 template <int L>
 __global__ __launch_bounds__(FftCfgS<L>::T) void corr_fund_kernel(
     const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    int step0, int off, const int* __restrict__ fullrlo, const int* __restrict__ rlo_h,
-    const float* __restrict__ sub, HarmSumArgs hs, float* __restrict__ plane,
-    fdas_candidate* __restrict__ cands, int* __restrict__ ncand, const cplx* __restrict__ tw,
-    float scale)
+    int step0, int nsteps, int G, int off, const int* __restrict__ fullrlo,
+    const int* __restrict__ rlo_h, const float* __restrict__ sub, HarmSumArgs hs,
+    float* __restrict__ plane, fdas_candidate* __restrict__ cands, int* __restrict__ ncand,
+    const cplx* __restrict__ tw, float scale)
 {
     constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
+    constexpr int E = F::E;
+    constexpr int LDS_FLOATS = 2 * F::LDS_ELEMS;
+    static_assert(E <= 32, "stage masks hold 32 outputs per thread");
     __shared__ cplx lds[F::LDS_ELEMS];
-    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-    const int gstep = step0 + s;
-    cplx v[F::E];
-    corr_ifft<L>(bank + (size_t)z * L, seg + s * seg_stride, tw, lds, v, t);
-
-    const long long q0 = 2LL * fullrlo[gstep];
+    float* ldsf = reinterpret_cast<float*>(lds);
+    const int z = blockIdx.x, t = threadIdx.x;
+    const int s0 = blockIdx.y * G, s1 = min(s0 + G, nsteps);
     const int zz = hs.zlo0 + DZ * z;
-    const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
-    float* o = plane + ((size_t)s * Z + z) * hs.U;
+    const int U = hs.U;
+    cplx kr[E], v[E];
+    load_template<L>(bank + (size_t)z * L, kr, t);
+    for (int s = s0; s < s1; ++s) {
+        const int gstep = step0 + s;
+        corr_ifft<L>(kr, seg + s * seg_stride, tw, lds, v, t);
+        const long long q0 = 2LL * fullrlo[gstep];
+        const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
+        float acc[E];
+        unsigned mask[5] = {0u, 0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) {
-        const int col = F::out_index(t, i) - off;
-        if (col < 0 || col >= hs.U) continue;
-        float acc = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
-        int stage = 0;
-        if (acc > hs.powcut[0]) {
-            const int idx = atomicAdd(ncand, 1);
-            if (idx < hs.cand_cap) {
-                fdas_candidate c;
-                c.step = gstep; c.z_index = z; c.r_index = col; c.stage = 0; c.power = acc;
-                c.reserved = 0.f;
-                cands[idx] = c;
-            }
+        for (int i = 0; i < E; ++i) {
+            const int col = F::out_index(t, i) - off;
+            acc[i] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+            if (col >= 0 && col < U && acc[i] > hs.powcut[0]) mask[0] |= 1u << i;
         }
-        const long long q = q0 + col;
-        for (int h = 1; h < hs.H; ++h) {
-            // subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm, rr = fullrlo + col/2
-            const int lm = hs.lm[h];
-            const long long sr2 = (q * hs.c[h] + (1LL << (lm - 1))) >> lm;
-            const int rind = (int)(sr2 - 2LL * rl[h]);
-            // subz = (int)(zz f) truncates toward zero; zind = (subz - zlo_h) / 2
-            const int subz = (zz * hs.c[h]) / (1 << lm);
-            const int zind = (subz - hs.zlo[h]) >> 1;
-            acc += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
-            if (((h + 1) & h) == 0) {
-                ++stage;
-                if (acc > hs.powcut[stage]) {
-                    const int idx = atomicAdd(ncand, 1);
-                    if (idx < hs.cand_cap) {
-                        fdas_candidate c;
-                        c.step = gstep; c.z_index = z; c.r_index = col; c.stage = stage;
-                        c.power = acc; c.reserved = 0.f;
-                        cands[idx] = c;
+        int stage = 0;
+        for (int h = 1; h < hs.H;) {
+            // harmonics [h, h2) whose row segments fit in LDS together
+            int h2 = h, used = 0;
+            while (h2 < hs.H) {
+                const int lm = hs.lm[h2], c = hs.c[h2], r2 = 2 * rl[h2];
+                const int len = sub_rind(q0 + U - 1, c, lm, r2) - sub_rind(q0, c, lm, r2) + 1;
+                if (used + len > LDS_FLOATS) break;
+                used += len;
+                ++h2;
+            }
+            __syncthreads();   // previous readers of the LDS buffer are done
+            int base = 0;
+            for (int g = h; g < h2; ++g) {
+                const int lm = hs.lm[g], c = hs.c[g], r2 = 2 * rl[g];
+                const int lo = sub_rind(q0, c, lm, r2);
+                const int len = sub_rind(q0 + U - 1, c, lm, r2) - lo + 1;
+                const int zind = ((zz * c) / (1 << lm) - hs.zlo[g]) >> 1;
+                const float* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
+                                   (long long)zind * hs.W[g] + lo;
+                for (int k = t; k < len; k += T) ldsf[base + k] = src[k];
+                base += len;
+            }
+            __syncthreads();
+            base = 0;
+            for (int g = h; g < h2; ++g) {
+                const int lm = hs.lm[g], c = hs.c[g], r2 = 2 * rl[g];
+                const int lo = sub_rind(q0, c, lm, r2);
+#pragma unroll
+                for (int i = 0; i < E; ++i) {
+                    const int col = F::out_index(t, i) - off;
+                    const int cc = col < 0 ? 0 : (col >= U ? U - 1 : col);
+                    acc[i] += ldsf[base + sub_rind(q0 + cc, c, lm, r2) - lo];
+                }
+                base += sub_rind(q0 + U - 1, c, lm, r2) - lo + 1;
+                if (((g + 1) & g) == 0) {
+                    ++stage;
+#pragma unroll
+                    for (int i = 0; i < E; ++i) {
+                        const int col = F::out_index(t, i) - off;
+                        if (col >= 0 && col < U && acc[i] > hs.powcut[stage]) mask[stage] |= 1u << i;
+                    }
+                }
+            }
+            h = h2;
+        }
+        float* o = plane + ((size_t)s * Z + z) * U;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int col = F::out_index(t, i) - off;
+            if (col >= 0 && col < U) o[col] = acc[i];
+        }
+        // rare path: candidates with their stage partial sums
+        if (mask[0] | mask[1] | mask[2] | mask[3] | mask[4]) {
+            for (int i = 0; i < E; ++i) {
+                unsigned any = 0;
+                for (int st = 0; st < 5; ++st) any |= (mask[st] >> i) & 1u;
+                if (!any) continue;
+                const int col = F::out_index(t, i) - off;
+                const float p0 = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+                float a2 = p0;
+                int st = 0;
+                if (mask[0] >> i & 1u) emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, 0, a2);
+                for (int h = 1; h < hs.H; ++h) {
+                    const int lm = hs.lm[h], c = hs.c[h];
+                    const int rind = sub_rind(q0 + col, c, lm, 2 * rl[h]);
+                    const int zind = ((zz * c) / (1 << lm) - hs.zlo[h]) >> 1;
+                    a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+                    if (((h + 1) & h) == 0) {
+                        ++st;
+                        if (mask[st] >> i & 1u)
+                            emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, st, a2);
                     }
                 }
             }
         }
-        o[col] = acc;
     }
 }
 
@@ -268,6 +369,8 @@ int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_
 {
     if (nseg <= 0) return FDAS_OK;
     switch (L / 2) {
+        case 128: return seg_prep_t<128>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 256: return seg_prep_t<256>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
         case 512: return seg_prep_t<512>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
         case 1024: return seg_prep_t<1024>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
         case 2048: return seg_prep_t<2048>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
@@ -279,6 +382,18 @@ int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_
     }
 }
 
+// Steps one workgroup walks with its template held in registers.  FDAS_STEPS_PER_WG
+// overrides the default (8).
+static int steps_per_group(int nsteps)
+{
+    static int g = [] {
+        const char* e = getenv("FDAS_STEPS_PER_WG");
+        const int v = e ? atoi(e) : 8;
+        return v > 0 ? v : 8;
+    }();
+    return g < nsteps ? g : (nsteps > 0 ? nsteps : 1);
+}
+
 template <int L>
 static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride,
                       long long seg_off, int nsteps, int off, int W, float* out, hipStream_t s)
@@ -287,8 +402,9 @@ static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_st
     const cplx* tw = fft_twiddles(L, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    hipLaunchKernelGGL(corr_sub_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
-                       seg, seg_stride, seg_off, off, W, out, tw, scale);
+    const int G = steps_per_group(nsteps);
+    hipLaunchKernelGGL(corr_sub_kernel<L>, dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T), 0, s,
+                       bank, Z, seg, seg_stride, seg_off, nsteps, G, off, W, out, tw, scale);
     return check_launch("corr_sub_kernel");
 }
 
@@ -297,6 +413,8 @@ int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long lo
 {
     if (nsteps <= 0) return FDAS_OK;
     switch (L) {
+        case 256: return corr_sub_t<256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 512: return corr_sub_t<512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 1024: return corr_sub_t<1024>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 2048: return corr_sub_t<2048>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 4096: return corr_sub_t<4096>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
@@ -318,9 +436,10 @@ static int corr_fund_t(const cplx* bank, int Z, const cplx* seg, long long seg_s
     const cplx* tw = fft_twiddles(L, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
-                       seg, seg_stride, step0, off, fullrlo, rlo_h, sub, hs, plane, cands, ncand,
-                       tw, scale);
+    const int G = steps_per_group(nsteps);
+    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T), 0,
+                       s, bank, Z, seg, seg_stride, step0, nsteps, G, off, fullrlo, rlo_h, sub, hs,
+                       plane, cands, ncand, tw, scale);
     return check_launch("corr_fund_kernel");
 }
 
@@ -331,6 +450,8 @@ int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long l
 {
     if (nsteps <= 0) return FDAS_OK;
     switch (L) {
+        case 256: return corr_fund_t<256>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
+        case 512: return corr_fund_t<512>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
         case 1024: return corr_fund_t<1024>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
         case 2048: return corr_fund_t<2048>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
         case 4096: return corr_fund_t<4096>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);

@@ -45,7 +45,8 @@ def test_c2c_fft(fd, n):
         out.free()
 
 
-@pytest.mark.parametrize("n", [256, 4096, 1 << 16, 1 << 20, 1 << 23])
+@pytest.mark.parametrize("n", [256, 4096, 1 << 16, 1 << 20, 1 << 23, 3 << 18, (1 << 20) + 2,
+                               2_000_006])
 def test_r2c_fft(fd, oracle, n):
     print_case("r2c", n)
     rng = np.random.default_rng(7 + n)
@@ -154,6 +155,15 @@ def test_plan_harmonics_4_16(fd, oracle):
     for numharm, fftlen in ((2, 1024), (4, 2048), (16, 4096)):
         print_case("plan numharm", numharm)
         _plan_vs_oracle(fd, oracle, x, n, 60, numharm, fftlen, [0, 3, 7])
+
+
+def test_plan_uneven_length(fd, oracle):
+    """Series lengths that are not powers of two (chirp-z R2C path)."""
+    from gpu_accelsearch_amd import synth
+    for n, steps in ((1_000_002, [0, 100, 500]), (3 << 19, [1, 700])):
+        x = synth.plain_series(n, seed=12, r0=30000.4, z0=-6.0, power=70.0)
+        print_case("plan uneven n", n)
+        _plan_vs_oracle(fd, oracle, x, n, 50, 2, 2048, steps)
 
 
 def test_stage_pipeline_matches_plan(fd, oracle):
