@@ -57,7 +57,7 @@ class FdasTiming(C.Structure):
     _fields_ = [("launches", C.c_int * 8), ("ms", C.c_double * 8)]
 
 
-KERNEL_CATEGORIES = ["r2c", "whiten", "seg_prep", "corr_sub", "corr_fund"]
+KERNEL_CATEGORIES = ["r2c", "whiten", "seg_prep", "corr_sub", "corr_fund", "harmsum"]
 
 
 class FdasCandidate(C.Structure):

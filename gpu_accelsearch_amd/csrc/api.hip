@@ -511,6 +511,9 @@ struct fdas_plan {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     size_t bytes = 0;
+    // harmonic sum inside the fundamental's correlation kernel (FDAS_HARMSUM=fused)
+    // instead of the tiled harmonic-sum kernel
+    bool fused_harmsum = false;
     // event timing
     bool timing = false;
     struct Rec {
@@ -804,6 +807,8 @@ extern "C" int fdas_plan_create(const fdas_params* params, fdas_plan** out)
     *out = nullptr;
     fdas_plan* pl = new fdas_plan();
     pl->p = *params;
+    const char* hsenv = getenv("FDAS_HARMSUM");
+    pl->fused_harmsum = hsenv && strcmp(hsenv, "fused") == 0;
     const int rc = plan_build(pl);
     if (rc != FDAS_OK) {
         fdas_plan_destroy(pl);
@@ -902,6 +907,17 @@ extern "C" int fdas_run_steps(fdas_plan* pl, int step0, int nsteps, float* d_pla
                                      pl->d_sub + pl->hs.sub_off[h], pl->stream));
         }
         float* plane = d_planes ? d_planes + (size_t)(c0 - step0) * pl->Z0 * U : pl->d_plane;
+        if (!pl->fused_harmsum) {
+            {
+                TimedLaunch tl(pl, FDAS_K_CORR_FUND);
+                FDAS_TRY(launch_corr_sub(pl->L[0], pl->bank[0], pl->Z0, pl->d_seg, pl->seg_stride, 0,
+                                         cs, pl->off[0], U, plane, pl->stream));
+            }
+            TimedLaunch tl(pl, FDAS_K_HARMSUM);
+            FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo,
+                                    pl->d_rlo_h, pl->d_cands, pl->d_ncand, pl->stream));
+            continue;
+        }
         TimedLaunch tl(pl, FDAS_K_CORR_FUND);
         FDAS_TRY(launch_corr_fund(pl->L[0], pl->bank[0], pl->Z0, pl->d_seg, pl->seg_stride, c0, cs,
                                   pl->off[0], pl->d_fullrlo, pl->d_rlo_h, pl->d_sub, pl->hs, plane,

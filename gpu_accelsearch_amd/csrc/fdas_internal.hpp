@@ -129,6 +129,12 @@ int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long l
                      const float* d_sub, const HarmSumArgs& hs, float* d_plane,
                      fdas_candidate* d_cands, int* d_ncand, hipStream_t s);
 
+// Harmonic sum in place over d_plane[step][Z0][U] (fundamental powers on entry),
+// stage thresholds and candidate emission.
+int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+                   int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
+                   int* d_ncand, hipStream_t s);
+
 // Stage-API kernels (unfused, reference data layout).
 int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,
                     hipStream_t s);

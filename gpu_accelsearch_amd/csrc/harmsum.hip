@@ -1,0 +1,218 @@
+// harmsum.hip -- harmonic summing over tiles of the fundamental f-fdot plane.
+//
+// Restates perform_harmonic_summing / addPowers (source/bpd_search.cu:310-360,
+// source/bpd_gpu_kernels.cu:119-170): for every fundamental cell (row j, column i)
+// P[j][i] += P_h[zind_h(j)][rind_h(i)] for h = 1..H-1 in that order, and the stage
+// thresholds after harmonics 1, 3, 7, 15 that the reference leaves as a stub
+// (source/bpd_search.cu:350-356).
+//
+// One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step.  The cells
+// of sub-harmonic h that the tile reads form a rectangle of about (HS_RJ f + 1) x
+// (HS_CB f + 1) values; the rectangles of as many harmonics as fit are staged in
+// LDS with coalesced loads (all loads of a harmonic issued before any store), and
+// the sum gathers from LDS.  Each fundamental cell thus costs about sum_h f_h^2
+// distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1 scattered ones.
+#include "fdas_internal.hpp"
+
+namespace fdas {
+
+namespace {
+constexpr int HS_T = 256;                                 // threads per tile
+constexpr int HS_RJ = 8;                                  // fundamental rows per tile
+constexpr int HS_CB = 512;                                // columns per tile
+constexpr int HS_CPT = HS_CB / HS_T;                      // columns per thread
+constexpr int HS_MAXR = HS_RJ + 1;                        // sub rows per harmonic, at most
+constexpr int HS_MAXK = (HS_CB + 2 + HS_T - 1) / HS_T;    // loads per sub row and thread
+constexpr int HS_LDS = 8192;                              // staged floats per round
+// one harmonic's rectangle, (HS_RJ f + 1) x (HS_CB f + 2) with f < 1, always fits
+static_assert(HS_MAXR * (HS_CB + 2) <= HS_LDS, "a single rectangle must fit the tile");
+}
+
+__device__ __forceinline__ int hs_rind(unsigned q, int c, int lm, int rlo2)
+{
+    return (int)((q * (unsigned)c + (1u << (lm - 1))) >> lm) - rlo2;
+}
+
+__device__ __forceinline__ int hs_zind(int zz, int c, int lm, int zlo_h)
+{
+    const int x = zz * c;
+    const int subz = x >= 0 ? (x >> lm) : -((-x) >> lm);
+    return (subz - zlo_h) >> 1;
+}
+
+__device__ __forceinline__ void hs_emit(fdas_candidate* cands, int* ncand, int cap, int step,
+                                        int z, int col, int stage, float p)
+{
+    const int idx = atomicAdd(ncand, 1);
+    if (idx < cap) {
+        fdas_candidate c;
+        c.step = step;
+        c.z_index = z;
+        c.r_index = col;
+        c.stage = stage;
+        c.power = p;
+        c.reserved = 0.f;
+        cands[idx] = c;
+    }
+}
+
+// grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
+// fundamental powers on entry and the summed powers on exit.
+__global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
+                                                       const float* __restrict__ sub,
+                                                       HarmSumArgs hs, int step0, int Z0,
+                                                       const int* __restrict__ fullrlo,
+                                                       const int* __restrict__ rlo_h,
+                                                       fdas_candidate* __restrict__ cands,
+                                                       int* __restrict__ ncand)
+{
+    __shared__ float tile[HS_LDS];
+    const int t = threadIdx.x;
+    const int c0 = blockIdx.x * HS_CB, j0 = blockIdx.y * HS_RJ, s = blockIdx.z;
+    const int gstep = step0 + s;
+    const int U = hs.U;
+    const int nrow = min(HS_RJ, Z0 - j0);
+    const int ncol = min(HS_CB, U - c0);
+    const unsigned q0 = 2u * (unsigned)fullrlo[gstep];
+    const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
+    float* base_row = plane + ((size_t)s * Z0 + j0) * U;
+
+    float acc[HS_RJ][HS_CPT], p0[HS_RJ][HS_CPT];
+    unsigned hit = 0;
+#pragma unroll
+    for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+        for (int k = 0; k < HS_CPT; ++k) {
+            const int col = c0 + t + k * HS_T;
+            const bool ok = r < nrow && col < c0 + ncol;
+            const float v = ok ? base_row[(size_t)r * U + col] : 0.f;
+            acc[r][k] = v;
+            p0[r][k] = v;
+            if (ok && v > hs.powcut[0]) hit |= 1u << (r * HS_CPT + k);
+        }
+
+    int stage = 0;
+    for (int h = 1; h < hs.H;) {
+        // harmonics [h, h2) whose rectangles fit in LDS together
+        int h2 = h, used = 0;
+        while (h2 < hs.H) {
+            const int c = hs.c[h2], lm = hs.lm[h2], r2 = 2 * rl[h2];
+            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[h2]) -
+                           hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[h2]) + 1;
+            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - hs_rind(q0 + c0, c, lm, r2) + 1;
+            if (used + nr * nc > HS_LDS) break;
+            used += nr * nc;
+            ++h2;
+        }
+        __syncthreads();   // earlier readers of the tile are done
+        int base = 0;
+        for (int g = h; g < h2; ++g) {
+            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
+            const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
+            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
+            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
+            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0 + 1;
+            const float* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
+                               (long long)zi0 * hs.W[g] + ri0;
+            float v[HS_MAXR][HS_MAXK];
+#pragma unroll
+            for (int rr = 0; rr < HS_MAXR; ++rr)
+#pragma unroll
+                for (int k = 0; k < HS_MAXK; ++k) {
+                    const int col = t + k * HS_T;
+                    v[rr][k] = (rr < nr && col < nc) ? src[(long long)rr * hs.W[g] + col] : 0.f;
+                }
+#pragma unroll
+            for (int rr = 0; rr < HS_MAXR; ++rr)
+#pragma unroll
+                for (int k = 0; k < HS_MAXK; ++k) {
+                    const int col = t + k * HS_T;
+                    if (rr < nr && col < nc) tile[base + rr * nc + col] = v[rr][k];
+                }
+            base += nr * nc;
+        }
+        __syncthreads();
+        base = 0;
+        for (int g = h; g < h2; ++g) {
+            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
+            const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
+            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
+            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
+            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0 + 1;
+            int ri[HS_CPT];
+#pragma unroll
+            for (int k = 0; k < HS_CPT; ++k) {
+                const int col = min(c0 + t + k * HS_T, c0 + ncol - 1);
+                ri[k] = hs_rind(q0 + col, c, lm, r2) - ri0;
+            }
+#pragma unroll
+            for (int r = 0; r < HS_RJ; ++r) {
+                const int zi = hs_zind(hs.zlo0 + DZ * (j0 + min(r, nrow - 1)), c, lm, hs.zlo[g]) - zi0;
+#pragma unroll
+                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[base + zi * nc + ri[k]];
+            }
+            base += nr * nc;
+            if (((g + 1) & g) == 0) {
+                ++stage;
+                const float pc = hs.powcut[stage];
+#pragma unroll
+                for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+                    for (int k = 0; k < HS_CPT; ++k)
+                        if (r < nrow && c0 + t + k * HS_T < c0 + ncol && acc[r][k] > pc)
+                            hit |= 1u << (r * HS_CPT + k);
+            }
+        }
+        h = h2;
+    }
+
+#pragma unroll
+    for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+        for (int k = 0; k < HS_CPT; ++k) {
+            const int col = c0 + t + k * HS_T;
+            if (r < nrow && col < c0 + ncol) base_row[(size_t)r * U + col] = acc[r][k];
+        }
+
+    // rare path: stage sums of the marked cells, re-accumulated in the same order
+    if (hit) {
+        for (int b = 0; b < HS_RJ * HS_CPT; ++b) {
+            if (!((hit >> b) & 1u)) continue;
+            const int r = b / HS_CPT, k = b % HS_CPT;
+            const int j = j0 + r, col = c0 + t + k * HS_T;
+            const int zz = hs.zlo0 + DZ * j;
+            float a2 = 0.f;
+#pragma unroll
+            for (int rr = 0; rr < HS_RJ; ++rr)
+#pragma unroll
+                for (int kk = 0; kk < HS_CPT; ++kk)
+                    if (rr == r && kk == k) a2 = p0[rr][kk];
+            int st = 0;
+            if (a2 > hs.powcut[0]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, 0, a2);
+            for (int h = 1; h < hs.H; ++h) {
+                const int c = hs.c[h], lm = hs.lm[h];
+                const int rind = hs_rind(q0 + col, c, lm, 2 * rl[h]);
+                const int zind = hs_zind(zz, c, lm, hs.zlo[h]);
+                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+                if (((h + 1) & h) == 0) {
+                    ++st;
+                    if (a2 > hs.powcut[st]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, st, a2);
+                }
+            }
+        }
+    }
+}
+
+int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+                   int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
+                   int* d_ncand, hipStream_t s)
+{
+    if (nsteps <= 0) return FDAS_OK;
+    dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
+              (unsigned)nsteps);
+    hipLaunchKernelGGL(harmsum_kernel, grid, dim3(HS_T), 0, s, d_plane, d_sub, hs, step0, Z0,
+                       d_fullrlo, d_rlo_h, d_cands, d_ncand);
+    return check_launch("harmsum_kernel");
+}
+
+}  // namespace fdas

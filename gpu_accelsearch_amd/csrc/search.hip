@@ -180,22 +180,32 @@ __device__ __forceinline__ void corr_ifft(const cplx* kr, const cplx* __restrict
     F::run(v, lds, tw, t);
 }
 
-// Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps)).
-template <int L>
+// Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps))
+// with the template row held in registers across them.
+template <int L, int G>
 __global__ __launch_bounds__(FftCfgS<L>::T) void corr_sub_kernel(
     const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    long long seg_off, int nsteps, int G, int off, int W, float* __restrict__ out,
+    long long seg_off, int nsteps, int off, int W, float* __restrict__ out,
     const cplx* __restrict__ tw, float scale)
 {
     constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
     __shared__ cplx lds[F::LDS_ELEMS];
     const int z = blockIdx.x, t = threadIdx.x;
-    const int s0 = blockIdx.y * G, s1 = min(s0 + G, nsteps);
-    cplx kr[F::E], v[F::E];
-    load_template<L>(bank + (size_t)z * L, kr, t);
+    const int s0 = blockIdx.y * G;
+    // G == 1: exactly one step (a loop would let the compiler hoist every twiddle)
+    const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
+    cplx v[F::E];
+    [[maybe_unused]] cplx kr[G > 1 ? F::E : 1];
+    if constexpr (G > 1) load_template<L>(bank + (size_t)z * L, kr, t);
     for (int s = s0; s < s1; ++s) {
-        corr_ifft<L>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+        if constexpr (G > 1) {
+            corr_ifft<L>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+        } else {
+            cplx k1[F::E];
+            load_template<L>(bank + (size_t)z * L, k1, t);
+            corr_ifft<L>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
+        }
         float* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
         for (int i = 0; i < F::E; ++i) {
@@ -222,125 +232,156 @@ __device__ __forceinline__ void emit_candidate(fdas_candidate* cands, int* ncand
 }
 
 // Sub-harmonic column read by fundamental column col (source/bpd_gpu_kernels.cu:139-141):
-// subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm and rr = fullrlo + col/2.
-__device__ __forceinline__ int sub_rind(long long q, int c, int lm, int rlo2)
+// subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm and q = 2 rr = 2 fullrlo + col.
+// q*c < 2^31 for series up to 2^27 samples and c < 16.
+__device__ __forceinline__ int sub_rind(unsigned q, int c, int lm, int rlo2)
 {
-    return (int)(((q * c + (1LL << (lm - 1))) >> lm) - rlo2);
+    return (int)((q * (unsigned)c + (1u << (lm - 1))) >> lm) - rlo2;
+}
+
+// Sub-harmonic row read by fundamental row zz = zlo + 2j: (int)(zz f) truncates
+// toward zero, then (subz - zlo_h) / 2 (source/bpd_gpu_kernels.cu:147-153).
+__device__ __forceinline__ int sub_zind(int zz, int c, int lm, int zlo_h)
+{
+    const int x = zz * c;
+    const int subz = x >= 0 ? (x >> lm) : -((-x) >> lm);
+    return (subz - zlo_h) >> 1;
 }
 
 // Fundamental: y = IFFT(A * conj K), power, then the harmonic sum in the reference
-// order.  The sub-harmonic row segments a workgroup needs (row zind_h, columns
-// rind(0)..rind(U-1), about U*f_h floats each) are staged in LDS with coalesced
-// loads, as many harmonics at a time as the FFT's LDS buffer holds, and gathered
-// from there.  Stage thresholds set per-stage bit masks; the rare candidates are
-// emitted afterwards (their partial sums recomputed from the sub planes).
+// order.  The sub-harmonic row segments this workgroup needs (row zind_h, columns
+// rind(0)..rind(U-1), about U*f_h floats each) are staged in the FFT's LDS buffer,
+// as many harmonics at a time as fit, with batches of independent loads, and
+// gathered from there.  Stage thresholds mark outputs in one bit mask; the rare
+// candidates are emitted afterwards with their partial sums recomputed.
 template <int L>
 __global__ __launch_bounds__(FftCfgS<L>::T) void corr_fund_kernel(
     const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    int step0, int nsteps, int G, int off, const int* __restrict__ fullrlo,
-    const int* __restrict__ rlo_h, const float* __restrict__ sub, HarmSumArgs hs,
-    float* __restrict__ plane, fdas_candidate* __restrict__ cands, int* __restrict__ ncand,
-    const cplx* __restrict__ tw, float scale)
+    int step0, int off, const int* __restrict__ fullrlo, const int* __restrict__ rlo_h,
+    const float* __restrict__ sub, HarmSumArgs hs, float* __restrict__ plane,
+    fdas_candidate* __restrict__ cands, int* __restrict__ ncand, const cplx* __restrict__ tw,
+    float scale)
 {
     constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
     constexpr int E = F::E;
     constexpr int LDS_FLOATS = 2 * F::LDS_ELEMS;
-    static_assert(E <= 32, "stage masks hold 32 outputs per thread");
+    constexpr int BATCH = 16;
+    static_assert(E <= 32, "the hit mask holds 32 outputs per thread");
     __shared__ cplx lds[F::LDS_ELEMS];
+    __shared__ long long meta_src[MAX_HARM];
+    __shared__ int meta_base[MAX_HARM + 1];
     float* ldsf = reinterpret_cast<float*>(lds);
-    const int z = blockIdx.x, t = threadIdx.x;
-    const int s0 = blockIdx.y * G, s1 = min(s0 + G, nsteps);
+    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+    const int gstep = step0 + s;
     const int zz = hs.zlo0 + DZ * z;
     const int U = hs.U;
-    cplx kr[E], v[E];
-    load_template<L>(bank + (size_t)z * L, kr, t);
-    for (int s = s0; s < s1; ++s) {
-        const int gstep = step0 + s;
+    cplx v[E];
+    {
+        cplx kr[E];
+        load_template<L>(bank + (size_t)z * L, kr, t);
         corr_ifft<L>(kr, seg + s * seg_stride, tw, lds, v, t);
-        const long long q0 = 2LL * fullrlo[gstep];
-        const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
-        float acc[E];
-        unsigned mask[5] = {0u, 0u, 0u, 0u, 0u};
+    }
+    const unsigned q0 = 2u * (unsigned)fullrlo[gstep];
+    const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
+    float acc[E], p0[E];
+    unsigned hit = 0;
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int col = F::out_index(t, i) - off;
-            acc[i] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
-            if (col >= 0 && col < U && acc[i] > hs.powcut[0]) mask[0] |= 1u << i;
+    for (int i = 0; i < E; ++i) {
+        const int col = F::out_index(t, i) - off;
+        p0[i] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+        acc[i] = p0[i];
+        if (col >= 0 && col < U && p0[i] > hs.powcut[0]) hit |= 1u << i;
+    }
+    int stage = 0;
+    for (int h = 1; h < hs.H;) {
+        // harmonics [h, h2) whose row segments fit in LDS together
+        int h2 = h, used = 0;
+        while (h2 < hs.H) {
+            const int r2 = 2 * rl[h2];
+            const int len = sub_rind(q0 + U - 1, hs.c[h2], hs.lm[h2], r2) -
+                            sub_rind(q0, hs.c[h2], hs.lm[h2], r2) + 1;
+            if (used + len > LDS_FLOATS) break;
+            used += len;
+            ++h2;
         }
-        int stage = 0;
-        for (int h = 1; h < hs.H;) {
-            // harmonics [h, h2) whose row segments fit in LDS together
-            int h2 = h, used = 0;
-            while (h2 < hs.H) {
-                const int lm = hs.lm[h2], c = hs.c[h2], r2 = 2 * rl[h2];
-                const int len = sub_rind(q0 + U - 1, c, lm, r2) - sub_rind(q0, c, lm, r2) + 1;
-                if (used + len > LDS_FLOATS) break;
-                used += len;
-                ++h2;
-            }
-            __syncthreads();   // previous readers of the LDS buffer are done
+        __syncthreads();   // earlier readers of the LDS buffer are done
+        if (t == 0) {
             int base = 0;
             for (int g = h; g < h2; ++g) {
-                const int lm = hs.lm[g], c = hs.c[g], r2 = 2 * rl[g];
-                const int lo = sub_rind(q0, c, lm, r2);
-                const int len = sub_rind(q0 + U - 1, c, lm, r2) - lo + 1;
-                const int zind = ((zz * c) / (1 << lm) - hs.zlo[g]) >> 1;
-                const float* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
-                                   (long long)zind * hs.W[g] + lo;
-                for (int k = t; k < len; k += T) ldsf[base + k] = src[k];
-                base += len;
+                const int r2 = 2 * rl[g];
+                const int lo = sub_rind(q0, hs.c[g], hs.lm[g], r2);
+                const int zind = sub_zind(zz, hs.c[g], hs.lm[g], hs.zlo[g]);
+                meta_src[g - h] = hs.sub_off[g] + s * hs.sub_stride[g] +
+                                  (long long)zind * hs.W[g] + lo - base;
+                meta_base[g - h] = base;
+                base += sub_rind(q0 + U - 1, hs.c[g], hs.lm[g], r2) - lo + 1;
             }
-            __syncthreads();
-            base = 0;
-            for (int g = h; g < h2; ++g) {
-                const int lm = hs.lm[g], c = hs.c[g], r2 = 2 * rl[g];
-                const int lo = sub_rind(q0, c, lm, r2);
+            meta_base[h2 - h] = base;
+        }
+        __syncthreads();
+        const int ng = h2 - h;
+        for (int k0 = t; k0 < used; k0 += BATCH * T) {
+            float r[BATCH];
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int k = k0 + u * T;
+                int g = 0;
+                while (g + 1 < ng && k >= meta_base[g + 1]) ++g;
+                r[u] = k < used ? sub[meta_src[g] + k] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < BATCH; ++u) {
+                const int k = k0 + u * T;
+                if (k < used) ldsf[k] = r[u];
+            }
+        }
+        __syncthreads();
+        for (int g = h; g < h2; ++g) {
+            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
+            const int shift = meta_base[g - h] - sub_rind(q0, c, lm, r2);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                int col = F::out_index(t, i) - off;
+                col = col < 0 ? 0 : (col >= U ? U - 1 : col);
+                acc[i] += ldsf[shift + sub_rind(q0 + col, c, lm, r2)];
+            }
+            if (((g + 1) & g) == 0) {
+                ++stage;
+                const float pc = hs.powcut[stage];
 #pragma unroll
                 for (int i = 0; i < E; ++i) {
                     const int col = F::out_index(t, i) - off;
-                    const int cc = col < 0 ? 0 : (col >= U ? U - 1 : col);
-                    acc[i] += ldsf[base + sub_rind(q0 + cc, c, lm, r2) - lo];
-                }
-                base += sub_rind(q0 + U - 1, c, lm, r2) - lo + 1;
-                if (((g + 1) & g) == 0) {
-                    ++stage;
-#pragma unroll
-                    for (int i = 0; i < E; ++i) {
-                        const int col = F::out_index(t, i) - off;
-                        if (col >= 0 && col < U && acc[i] > hs.powcut[stage]) mask[stage] |= 1u << i;
-                    }
+                    if (col >= 0 && col < U && acc[i] > pc) hit |= 1u << i;
                 }
             }
-            h = h2;
         }
-        float* o = plane + ((size_t)s * Z + z) * U;
+        h = h2;
+    }
+    float* o = plane + ((size_t)s * Z + z) * U;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        const int col = F::out_index(t, i) - off;
+        if (col >= 0 && col < U) o[col] = acc[i];
+    }
+    // rare path: the marked outputs' stage sums, re-accumulated in the same order
+    if (hit) {
 #pragma unroll
         for (int i = 0; i < E; ++i) {
+            if (!((hit >> i) & 1u)) continue;
             const int col = F::out_index(t, i) - off;
-            if (col >= 0 && col < U) o[col] = acc[i];
-        }
-        // rare path: candidates with their stage partial sums
-        if (mask[0] | mask[1] | mask[2] | mask[3] | mask[4]) {
-            for (int i = 0; i < E; ++i) {
-                unsigned any = 0;
-                for (int st = 0; st < 5; ++st) any |= (mask[st] >> i) & 1u;
-                if (!any) continue;
-                const int col = F::out_index(t, i) - off;
-                const float p0 = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
-                float a2 = p0;
-                int st = 0;
-                if (mask[0] >> i & 1u) emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, 0, a2);
-                for (int h = 1; h < hs.H; ++h) {
-                    const int lm = hs.lm[h], c = hs.c[h];
-                    const int rind = sub_rind(q0 + col, c, lm, 2 * rl[h]);
-                    const int zind = ((zz * c) / (1 << lm) - hs.zlo[h]) >> 1;
-                    a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
-                    if (((h + 1) & h) == 0) {
-                        ++st;
-                        if (mask[st] >> i & 1u)
-                            emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, st, a2);
-                    }
+            float a2 = p0[i];
+            int st = 0;
+            if (a2 > hs.powcut[0]) emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, 0, a2);
+            for (int h = 1; h < hs.H; ++h) {
+                const int c = hs.c[h], lm = hs.lm[h];
+                const int rind = sub_rind(q0 + col, c, lm, 2 * rl[h]);
+                const int zind = sub_zind(zz, c, lm, hs.zlo[h]);
+                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+                if (((h + 1) & h) == 0) {
+                    ++st;
+                    if (a2 > hs.powcut[st])
+                        emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, st, a2);
                 }
             }
         }
@@ -382,18 +423,6 @@ int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_
     }
 }
 
-// Steps one workgroup walks with its template held in registers.  FDAS_STEPS_PER_WG
-// overrides the default (8).
-static int steps_per_group(int nsteps)
-{
-    static int g = [] {
-        const char* e = getenv("FDAS_STEPS_PER_WG");
-        const int v = e ? atoi(e) : 8;
-        return v > 0 ? v : 8;
-    }();
-    return g < nsteps ? g : (nsteps > 0 ? nsteps : 1);
-}
-
 template <int L>
 static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride,
                       long long seg_off, int nsteps, int off, int W, float* out, hipStream_t s)
@@ -402,9 +431,11 @@ static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_st
     const cplx* tw = fft_twiddles(L, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    const int G = steps_per_group(nsteps);
-    hipLaunchKernelGGL(corr_sub_kernel<L>, dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T), 0, s,
-                       bank, Z, seg, seg_stride, seg_off, nsteps, G, off, W, out, tw, scale);
+    // Templates stay in registers across 8 steps up to length 8192; at 16384 the
+    // registers are needed for the transform itself.
+    constexpr int G = L <= 8192 ? 8 : 1;
+    hipLaunchKernelGGL((corr_sub_kernel<L, G>), dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T),
+                       0, s, bank, Z, seg, seg_stride, seg_off, nsteps, off, W, out, tw, scale);
     return check_launch("corr_sub_kernel");
 }
 
@@ -436,10 +467,9 @@ static int corr_fund_t(const cplx* bank, int Z, const cplx* seg, long long seg_s
     const cplx* tw = fft_twiddles(L, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    const int G = steps_per_group(nsteps);
-    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T), 0,
-                       s, bank, Z, seg, seg_stride, step0, nsteps, G, off, fullrlo, rlo_h, sub, hs,
-                       plane, cands, ncand, tw, scale);
+    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
+                       seg, seg_stride, step0, off, fullrlo, rlo_h, sub, hs, plane, cands, ncand,
+                       tw, scale);
     return check_launch("corr_fund_kernel");
 }
 

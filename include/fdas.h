@@ -217,8 +217,9 @@ enum {
     FDAS_K_WHITEN = 1,     /* block medians + normalisation                    */
     FDAS_K_SEGPREP = 2,    /* segment fetch, median, normalise, forward FFT    */
     FDAS_K_CORR_SUB = 3,   /* fused correlation/IFFT/power, sub-harmonics      */
-    FDAS_K_CORR_FUND = 4,  /* fused correlation/IFFT/power + harmonic sum      */
-    FDAS_K_NCAT = 5
+    FDAS_K_CORR_FUND = 4,  /* fused correlation/IFFT/power, fundamental        */
+    FDAS_K_HARMSUM = 5,    /* tiled harmonic sum + stage thresholds            */
+    FDAS_K_NCAT = 6
 };
 typedef struct fdas_timing {
     int launches[8];
