@@ -76,6 +76,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps sampled")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB "
+                         "(a kernel of known traffic for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -156,6 +159,13 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(series, n, zmax, numharm, fftlen, args.cpu_steps)
         out["cpu_baseline"]["vs_gpu"] = cells_per_s / out["cpu_baseline"]["value"]
+    if args.copy_probe_mib > 0:
+        from gpu_accelsearch_amd import lib
+        nb = args.copy_probe_mib << 20
+        a, b = DeviceArray((nb,), np.uint8), DeviceArray((nb,), np.uint8)
+        lib().fdas_memcpy_dtod(b.ptr, a.ptr, nb)
+        synchronize()
+        out["copy_probe_bytes"] = nb
     s.close()
     print(json.dumps(out), flush=True)
     return 0

@@ -121,6 +121,7 @@ PROTOTYPES = [
     ("fdas_free", i32, [vp]),
     ("fdas_memcpy_htod", i32, [vp, vp, C.c_size_t]),
     ("fdas_memcpy_dtoh", i32, [vp, vp, C.c_size_t]),
+    ("fdas_memcpy_dtod", i32, [vp, vp, C.c_size_t]),
     ("fdas_memset", i32, [vp, i32, C.c_size_t]),
     ("fdas_device_synchronize", i32, []),
     ("fdas_last_error", C.c_char_p, []),

@@ -97,6 +97,11 @@ extern "C" int fdas_memcpy_dtoh(void* h, const void* d, size_t bytes)
     FDAS_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
     return FDAS_OK;
 }
+extern "C" int fdas_memcpy_dtod(void* d, const void* src, size_t bytes)
+{
+    FDAS_HIP(hipMemcpy(d, src, bytes, hipMemcpyDeviceToDevice));
+    return FDAS_OK;
+}
 extern "C" int fdas_memset(void* d, int value, size_t bytes)
 {
     FDAS_HIP(hipMemset(d, value, bytes));

@@ -36,6 +36,8 @@ int fft_threads(int n);
 bool fft_row_supported(long n);
 // Device twiddle table of the row transform of length n (cached per device).
 const cplx* fft_twiddles(int n, int* err);
+// Same for a transform run by t threads (the pass radices depend on n/t).
+const cplx* fft_twiddles_t(int n, int t, int* err);
 
 // Row transforms: out[row*out_stride + k] = scale * DFT_dir(in[row*in_stride + .])[k]
 // (times exp(dir 2 pi i (row*k mod tw_m)/tw_m) when tw_m > 0).  In place allowed.

@@ -44,17 +44,22 @@ int fft_threads(int n)
 
 bool fft_row_supported(long n) { return fft_threads((int)n) > 0 && n <= 16384; }
 
-template <int N>
+template <int N, int T = FftCfg<N>::T>
 static std::vector<cplx> host_twiddles()
 {
-    using F = BlockFFT<N, FftCfg<N>::T, 1>;
+    using F = BlockFFT<N, T, 1>;
     std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1, mk(1.f, 0.f));
     F::build_twiddles(tw.data());
     return tw;
 }
 
-static std::vector<cplx> host_twiddles_for(int n)
+static std::vector<cplx> host_twiddles_for(int n, int t)
 {
+    if (t != fft_threads(n)) {
+        if (n == 8192 && t == 512) return host_twiddles<8192, 512>();
+        if (n == 4096 && t == 512) return host_twiddles<4096, 512>();
+        return {};
+    }
     switch (n) {
         case 64: return host_twiddles<64>();
         case 128: return host_twiddles<128>();
@@ -70,7 +75,9 @@ static std::vector<cplx> host_twiddles_for(int n)
 }
 
 // Twiddle tables live for the life of the process, one set per device.
-const cplx* fft_twiddles(int n, int* err)
+const cplx* fft_twiddles(int n, int* err) { return fft_twiddles_t(n, fft_threads(n), err); }
+
+const cplx* fft_twiddles_t(int n, int t, int* err)
 {
     static std::mutex mu;
     static std::map<std::pair<int, int>, cplx*> cache;
@@ -81,12 +88,12 @@ const cplx* fft_twiddles(int n, int* err)
         return nullptr;
     }
     std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_pair(dev, n);
+    auto key = std::make_pair(dev, n * 4096 + t);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    std::vector<cplx> h = host_twiddles_for(n);
+    std::vector<cplx> h = host_twiddles_for(n, t);
     if (h.empty()) {
-        set_error("no row FFT of length %d", n);
+        set_error("no row FFT of length %d with %d threads", n, t);
         *err = FDAS_ERR_UNSUPPORTED;
         return nullptr;
     }

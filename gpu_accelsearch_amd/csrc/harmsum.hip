@@ -6,27 +6,25 @@
 // thresholds after harmonics 1, 3, 7, 15 that the reference leaves as a stub
 // (source/bpd_search.cu:350-356).
 //
-// One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step.  The cells
+// One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step (2 x 2048).  The cells
 // of sub-harmonic h that the tile reads form a rectangle of about (HS_RJ f + 1) x
 // (HS_CB f + 1) values; the rectangles of as many harmonics as fit are staged in
 // LDS with coalesced loads (all loads of a harmonic issued before any store), and
 // the sum gathers from LDS.  Each fundamental cell thus costs about sum_h f_h^2
 // distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1 scattered ones.
+#include <cstdlib>
+
 #include "fdas_internal.hpp"
 
 namespace fdas {
 
-namespace {
-constexpr int HS_T = 256;                                 // threads per tile
-constexpr int HS_RJ = 8;                                  // fundamental rows per tile
-constexpr int HS_CB = 512;                                // columns per tile
-constexpr int HS_CPT = HS_CB / HS_T;                      // columns per thread
-constexpr int HS_MAXR = HS_RJ + 1;                        // sub rows per harmonic, at most
-constexpr int HS_MAXK = (HS_CB + 2 + HS_T - 1) / HS_T;    // loads per sub row and thread
-constexpr int HS_LDS = 8192;                              // staged floats per round
-// one harmonic's rectangle, (HS_RJ f + 1) x (HS_CB f + 2) with f < 1, always fits
-static_assert(HS_MAXR * (HS_CB + 2) <= HS_LDS, "a single rectangle must fit the tile");
-}
+// Tile shapes: HS_T threads, HS_RJ rows x HS_CB columns, HS_LDS staged floats.
+// HS_MAXR sub rows and HS_MAXK loads per sub row and thread bound one rectangle.
+#define HS_TILE_CONSTANTS                                                      \
+    constexpr int HS_CPT = HS_CB / HS_T;                                       \
+    constexpr int HS_MAXR = HS_RJ + 1;                                         \
+    constexpr int HS_MAXK = (HS_CB + 2 + HS_T - 1) / HS_T;                     \
+    static_assert(HS_MAXR * (HS_CB + 2) <= HS_LDS, "one rectangle must fit");
 
 __device__ __forceinline__ int hs_rind(unsigned q, int c, int lm, int rlo2)
 {
@@ -58,6 +56,7 @@ __device__ __forceinline__ void hs_emit(fdas_candidate* cands, int* ncand, int c
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
 // fundamental powers on entry and the summed powers on exit.
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
 __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
                                                        const float* __restrict__ sub,
                                                        HarmSumArgs hs, int step0, int Z0,
@@ -66,6 +65,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                                                        fdas_candidate* __restrict__ cands,
                                                        int* __restrict__ ncand)
 {
+    HS_TILE_CONSTANTS
     __shared__ float tile[HS_LDS];
     const int t = threadIdx.x;
     const int c0 = blockIdx.x * HS_CB, j0 = blockIdx.y * HS_RJ, s = blockIdx.z;
@@ -77,19 +77,31 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
     float* base_row = plane + ((size_t)s * Z0 + j0) * U;
 
+    // fundamental powers; their threshold test waits until the first round's
+    // loads are in flight too, so both arrive behind one memory wait
     float acc[HS_RJ][HS_CPT], p0[HS_RJ][HS_CPT];
     unsigned hit = 0;
+    bool first = true;
 #pragma unroll
     for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
         for (int k = 0; k < HS_CPT; ++k) {
             const int col = c0 + t + k * HS_T;
             const bool ok = r < nrow && col < c0 + ncol;
-            const float v = ok ? base_row[(size_t)r * U + col] : 0.f;
-            acc[r][k] = v;
-            p0[r][k] = v;
-            if (ok && v > hs.powcut[0]) hit |= 1u << (r * HS_CPT + k);
+            acc[r][k] = ok ? base_row[(size_t)r * U + col] : 0.f;
         }
+    auto mark_fundamental = [&]() {
+#pragma unroll
+        for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+            for (int k = 0; k < HS_CPT; ++k) {
+                p0[r][k] = acc[r][k];
+                const int col = c0 + t + k * HS_T;
+                if (r < nrow && col < c0 + ncol && acc[r][k] > hs.powcut[0])
+                    hit |= 1u << (r * HS_CPT + k);
+            }
+    };
+    if (hs.H <= 1) mark_fundamental();
 
     int stage = 0;
     for (int h = 1; h < hs.H;) {
@@ -122,6 +134,10 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                     const int col = t + k * HS_T;
                     v[rr][k] = (rr < nr && col < nc) ? src[(long long)rr * hs.W[g] + col] : 0.f;
                 }
+            if (first) {
+                mark_fundamental();
+                first = false;
+            }
 #pragma unroll
             for (int rr = 0; rr < HS_MAXR; ++rr)
 #pragma unroll
@@ -203,16 +219,33 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     }
 }
 
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
+static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0,
+                     int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
+                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
+{
+    dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
+              (unsigned)nsteps);
+    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS>), grid, dim3(HS_T), 0, s, d_plane,
+                       d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand);
+    return check_launch("harmsum_kernel");
+}
+
 int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s)
 {
     if (nsteps <= 0) return FDAS_OK;
-    dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
-              (unsigned)nsteps);
-    hipLaunchKernelGGL(harmsum_kernel, grid, dim3(HS_T), 0, s, d_plane, d_sub, hs, step0, Z0,
-                       d_fullrlo, d_rlo_h, d_cands, d_ncand);
-    return check_launch("harmsum_kernel");
+    // Tile shape: 2 rows x 2048 columns, 256 threads, 32 KB of LDS measured
+    // fastest of the shapes tried (long contiguous sub-harmonic runs matter more
+    // than the staged volume).  FDAS_HS_TILE=1 selects 8 x 512.
+    static const int tile = [] {
+        const char* e = getenv("FDAS_HS_TILE");
+        return e ? atoi(e) : 0;
+    }();
+    if (tile == 1)
+        return harmsum_t<256, 8, 512, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
+    return harmsum_t<256, 2, 2048, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
 }
 
 }  // namespace fdas

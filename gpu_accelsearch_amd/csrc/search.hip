@@ -154,10 +154,9 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
 
 // Template values of row z at this thread's first-pass positions, kept in
 // registers while the workgroup walks its group of steps.
-template <int L>
+template <int L, int T = FftCfgS<L>::T>
 __device__ __forceinline__ void load_template(const cplx* __restrict__ K, cplx* kr, int t)
 {
-    constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
 #pragma unroll
     for (int i = 0; i < F::E; ++i) kr[i] = K[F::in_index(t, i)];
@@ -165,11 +164,10 @@ __device__ __forceinline__ void load_template(const cplx* __restrict__ K, cplx* 
 
 // y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.  Position t + i*T needs
 // A[t + (i mod E/2)*T], so each thread loads E/2 segment values.
-template <int L>
+template <int L, int T = FftCfgS<L>::T>
 __device__ __forceinline__ void corr_ifft(const cplx* kr, const cplx* __restrict__ A,
                                           const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
 {
-    constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
     constexpr int EH = F::E / 2;
     cplx a[EH];
@@ -182,13 +180,12 @@ __device__ __forceinline__ void corr_ifft(const cplx* kr, const cplx* __restrict
 
 // Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps))
 // with the template row held in registers across them.
-template <int L, int G>
-__global__ __launch_bounds__(FftCfgS<L>::T) void corr_sub_kernel(
+template <int L, int G, int T>
+__global__ __launch_bounds__(T) void corr_sub_kernel(
     const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
     long long seg_off, int nsteps, int off, int W, float* __restrict__ out,
     const cplx* __restrict__ tw, float scale)
 {
-    constexpr int T = FftCfgS<L>::T;
     using F = BlockFFT<L, T, 1>;
     __shared__ cplx lds[F::LDS_ELEMS];
     const int z = blockIdx.x, t = threadIdx.x;
@@ -197,14 +194,14 @@ __global__ __launch_bounds__(FftCfgS<L>::T) void corr_sub_kernel(
     const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
     cplx v[F::E];
     [[maybe_unused]] cplx kr[G > 1 ? F::E : 1];
-    if constexpr (G > 1) load_template<L>(bank + (size_t)z * L, kr, t);
+    if constexpr (G > 1) load_template<L, T>(bank + (size_t)z * L, kr, t);
     for (int s = s0; s < s1; ++s) {
         if constexpr (G > 1) {
-            corr_ifft<L>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
         } else {
             cplx k1[F::E];
-            load_template<L>(bank + (size_t)z * L, k1, t);
-            corr_ifft<L>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            load_template<L, T>(bank + (size_t)z * L, k1, t);
+            corr_ifft<L, T>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
         }
         float* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
@@ -423,20 +420,29 @@ int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_
     }
 }
 
-template <int L>
+template <int L, int T = FftCfgS<L>::T>
 static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride,
                       long long seg_off, int nsteps, int off, int W, float* out, hipStream_t s)
 {
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles(L, &err);
+    const cplx* tw = fft_twiddles_t(L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
     // Templates stay in registers across 8 steps up to length 8192; at 16384 the
     // registers are needed for the transform itself.
     constexpr int G = L <= 8192 ? 8 : 1;
-    hipLaunchKernelGGL((corr_sub_kernel<L, G>), dim3(Z, (nsteps + G - 1) / G), dim3(FftCfgS<L>::T),
-                       0, s, bank, Z, seg, seg_stride, seg_off, nsteps, off, W, out, tw, scale);
+    hipLaunchKernelGGL((corr_sub_kernel<L, G, T>), dim3(Z, (nsteps + G - 1) / G), dim3(T), 0, s,
+                       bank, Z, seg, seg_stride, seg_off, nsteps, off, W, out, tw, scale);
     return check_launch("corr_sub_kernel");
+}
+
+// Threads of the fused correlation at each length: 512 (16 values per thread) at
+// 4096 and 8192 measured faster than 256 (more waves per SIMD outweigh the extra
+// LDS pass).  FDAS_CORR8192_T / FDAS_CORR4096_T=256 select the 256-thread form.
+static bool corr_t256(const char* var)
+{
+    const char* e = getenv(var);
+    return e && atoi(e) == 256;
 }
 
 int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
@@ -448,8 +454,14 @@ int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long lo
         case 512: return corr_sub_t<512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 1024: return corr_sub_t<1024>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 2048: return corr_sub_t<2048>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 4096: return corr_sub_t<4096>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 8192: return corr_sub_t<8192>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 4096:
+            if (corr_t256("FDAS_CORR4096_T"))
+                return corr_sub_t<4096, 256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+            return corr_sub_t<4096, 512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 8192:
+            if (corr_t256("FDAS_CORR8192_T"))
+                return corr_sub_t<8192, 256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+            return corr_sub_t<8192, 512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         case 16384: return corr_sub_t<16384>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
         default:
             set_error("fused correlation length %d unsupported", L);

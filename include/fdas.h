@@ -235,6 +235,7 @@ int fdas_malloc(void **d_ptr, size_t bytes);
 int fdas_free(void *d_ptr);
 int fdas_memcpy_htod(void *d_dst, const void *h_src, size_t bytes);
 int fdas_memcpy_dtoh(void *h_dst, const void *d_src, size_t bytes);
+int fdas_memcpy_dtod(void *d_dst, const void *d_src, size_t bytes);
 int fdas_memset(void *d_dst, int value, size_t bytes);
 int fdas_device_synchronize(void);
 const char *fdas_last_error(void);
