@@ -76,6 +76,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps sampled")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+                    help="storage of spectra segments and templates (arithmetic is FP32)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling profiler byte counters)")
@@ -91,7 +93,8 @@ def main() -> int:
     n, zmax, numharm, fftlen = 1 << 23, 200, 8, 8192
     series = synth.production_series(n)
     params = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen,
-                          chunk_steps=args.chunk, powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+                          chunk_steps=args.chunk, precision=args.precision,
+                          powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
     s = AccelSearch(params)
     d_series = DeviceArray.from_host(series)
     for _ in range(args.warmup):
@@ -140,6 +143,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
+        "storage": args.precision,
         "data": "synthetic: N(0,1) + red noise + 3 accelerated Gaussian-pulse pulsars, seed 2",
         "config": {"workload": "production DM trial: 2^23 samples, dt 64 us, zmax 200 "
                                "(201 templates), fftlen 8192 (uselen 7734), numharm 8, "

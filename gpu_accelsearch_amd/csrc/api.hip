@@ -492,7 +492,11 @@ struct fdas_plan {
     long nbins = 0;
     int cur[MAX_HARM]{}, mx[MAX_HARM]{}, L[MAX_HARM]{}, Z[MAX_HARM]{}, zmax_h[MAX_HARM]{},
         off[MAX_HARM]{}, W[MAX_HARM]{}, zlo[MAX_HARM]{};
-    cplx* bank[MAX_HARM]{};
+    cplx* bank[MAX_HARM]{};      // FP32 banks (freed after packing in reduced precision)
+    void* bankq[MAX_HARM]{};     // bank in the storage format used by the kernels
+    int* bank_exp[MAX_HARM]{};   // per-row exponents (reduced precision)
+    int prec = FDAS_PREC_FP32;
+    size_t cbytes = sizeof(cplx);   // bytes per stored complex value
     long long seg_off[MAX_HARM]{};
     long long seg_stride = 0;
     long long sub_total = 0;     // elements per step over all sub-harmonics
@@ -506,8 +510,10 @@ struct fdas_plan {
     cplx* d_r2c_work = nullptr;
     R2CPlan r2c;
     WhitenSchedule ws;
-    cplx* d_seg = nullptr;
-    float* d_sub = nullptr;
+    void* d_seg = nullptr;       // [chunk][seg_stride] stored complex values
+    int* d_seg_exp = nullptr;    // [chunk][H] segment exponents (reduced precision)
+    void* d_sub = nullptr;       // sub-harmonic planes, float or FP16
+    size_t sub_bytes = sizeof(float);
     float* d_plane = nullptr;
     fdas_candidate* d_cands = nullptr;
     int* d_ncand = nullptr;
@@ -577,10 +583,19 @@ static int plan_build(fdas_plan* pl)
         set_error("fftlen %d must be a power of two in [1024, 16384]", p.fftlen);
         return FDAS_ERR_UNSUPPORTED;
     }
-    if (p.precision != FDAS_PREC_FP32) {
-        set_error("precision mode %d not available", p.precision);
+    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_BF16) {
+        set_error("precision mode %d unknown", p.precision);
+        return FDAS_ERR_INVALID;
+    }
+    if (p.precision != FDAS_PREC_FP32 && pl->fused_harmsum) {
+        set_error("the fused harmonic-sum variant runs in FP32 only");
         return FDAS_ERR_UNSUPPORTED;
     }
+    pl->prec = p.precision;
+    pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : 4;
+    // power planes stay FP32: FP16 planes halve the bytes per staging load in the
+    // harmonic sum and measured slower
+    pl->sub_bytes = sizeof(float);
     pl->uselen = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
     if (pl->uselen <= 0 || (pl->uselen & 1) || pl->uselen >= 65536 ||
         h_fft_length(pl->uselen, 1, 1, p.zmax) != p.fftlen) {
@@ -720,6 +735,18 @@ static int plan_build(fdas_plan* pl)
     for (int h = 0; h < H; ++h) {
         FDAS_TRY(plan_alloc(pl, (void**)&pl->bank[h], (size_t)pl->Z[h] * pl->L[h] * sizeof(cplx)));
         FDAS_TRY(build_template_bank(pl->bank[h], pl->zmax_h[h], pl->Z[h], pl->L[h], pl->stream));
+        if (pl->prec == FDAS_PREC_FP32) {
+            pl->bankq[h] = pl->bank[h];
+        } else {
+            FDAS_TRY(plan_alloc(pl, &pl->bankq[h], (size_t)pl->Z[h] * pl->L[h] * pl->cbytes));
+            FDAS_TRY(plan_alloc(pl, (void**)&pl->bank_exp[h], (size_t)pl->Z[h] * sizeof(int)));
+            FDAS_TRY(launch_bank_pack(pl->bank[h], pl->Z[h], pl->L[h], pl->prec, pl->bankq[h],
+                                      pl->bank_exp[h], pl->stream));
+            FDAS_HIP(hipStreamSynchronize(pl->stream));
+            FDAS_HIP(hipFree(pl->bank[h]));
+            pl->bytes -= (size_t)pl->Z[h] * pl->L[h] * sizeof(cplx);
+            pl->bank[h] = nullptr;
+        }
     }
     for (size_t c = 0; c < pl->classes.size(); ++c) {
         const auto& hl = pl->class_harm[c];
@@ -731,6 +758,8 @@ static int plan_build(fdas_plan* pl)
                 e.lobin = lob[(size_t)s * H + h];
                 e.numdata = nd_[(size_t)s * H + h];
                 e.out = (long long)s * pl->seg_stride + pl->seg_off[h];
+                e.eslot = s * H + h;
+                e.pad = 0;
             }
         SegDesc* dd = nullptr;
         FDAS_TRY(plan_alloc(pl, (void**)&dd, d.size() * sizeof(SegDesc)));
@@ -745,9 +774,10 @@ static int plan_build(fdas_plan* pl)
     FDAS_TRY(r2c_plan_create(p.n, &pl->r2c, pl->stream));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_r2c_work, pl->r2c.work_elems * sizeof(cplx)));
     FDAS_TRY(whiten_schedule_create((unsigned)pl->nbins, &pl->ws));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg, (size_t)pl->chunk * pl->seg_stride * sizeof(cplx)));
+    FDAS_TRY(plan_alloc(pl, &pl->d_seg, (size_t)pl->chunk * pl->seg_stride * pl->cbytes));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg_exp, (size_t)pl->chunk * H * sizeof(int)));
     if (sub > 0)
-        FDAS_TRY(plan_alloc(pl, (void**)&pl->d_sub, (size_t)pl->chunk * sub * sizeof(float)));
+        FDAS_TRY(plan_alloc(pl, &pl->d_sub, (size_t)pl->chunk * sub * pl->sub_bytes));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_plane, (size_t)pl->chunk * pl->Z0 * U * sizeof(float)));
     pl->cand_cap = p.max_candidates > 0 ? p.max_candidates : (1 << 20);
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_cands, (size_t)pl->cand_cap * sizeof(fdas_candidate)));
@@ -774,6 +804,7 @@ static int plan_build(fdas_plan* pl)
     }
     for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
     hs.cand_cap = pl->cand_cap;
+    hs.sub_half = pl->sub_bytes == 2;
     FDAS_HIP(hipStreamSynchronize(pl->stream));
     return FDAS_OK;
 }
@@ -782,7 +813,11 @@ extern "C" void fdas_plan_destroy(fdas_plan* pl)
 {
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    for (int h = 0; h < MAX_HARM; ++h) (void)hipFree(pl->bank[h]);
+    for (int h = 0; h < MAX_HARM; ++h) {
+        if (pl->bankq[h] != pl->bank[h]) (void)hipFree(pl->bankq[h]);
+        (void)hipFree(pl->bank[h]);
+        (void)hipFree(pl->bank_exp[h]);
+    }
     for (SegDesc* d : pl->d_desc) (void)hipFree(d);
     (void)hipFree(pl->d_fullrlo);
     (void)hipFree(pl->d_rlo_h);
@@ -791,6 +826,7 @@ extern "C" void fdas_plan_destroy(fdas_plan* pl)
     r2c_plan_destroy(&pl->r2c);
     if (pl->ws.d_start) whiten_schedule_destroy(&pl->ws);
     (void)hipFree(pl->d_seg);
+    (void)hipFree(pl->d_seg_exp);
     (void)hipFree(pl->d_sub);
     (void)hipFree(pl->d_plane);
     (void)hipFree(pl->d_cands);
@@ -899,24 +935,50 @@ extern "C" int fdas_run_steps(fdas_plan* pl, int step0, int nsteps, float* d_pla
         const int cs = std::min(pl->chunk, step0 + nsteps - c0);
         for (size_t c = 0; c < pl->classes.size(); ++c) {
             const int nh = (int)pl->class_harm[c].size();
-            const SegDesc* desc = pl->d_desc[c] + (size_t)c0 * nh;
+            SegPrepArgs a{};
+            a.spec = pl->d_spec;
+            a.nbins = pl->nbins;
+            a.desc = pl->d_desc[c] + (size_t)c0 * nh;
+            a.nseg = cs * nh;
+            a.seg = pl->d_seg;
             // descriptors hold offsets as if every step had a buffer slot
+            a.out_base = (long long)c0 * pl->seg_stride;
+            a.seg_exp = pl->d_seg_exp;
+            a.eslot_base = c0 * pl->H;
+            a.prec = pl->prec;
             TimedLaunch tl(pl, FDAS_K_SEGPREP);
-            FDAS_TRY(launch_segment_prep(pl->classes[c], pl->d_spec, pl->nbins, desc, cs * nh,
-                                         pl->d_seg, (long long)c0 * pl->seg_stride, pl->stream));
+            FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
         }
+        auto corr_args = [&](int h, void* out, int W, int out_half) {
+            CorrArgs a{};
+            a.bank = pl->bankq[h];
+            a.bank_exp = pl->bank_exp[h];
+            a.Z = pl->Z[h];
+            a.seg = pl->d_seg;
+            a.seg_stride = pl->seg_stride;
+            a.seg_off = pl->seg_off[h];
+            a.seg_exp = pl->d_seg_exp;
+            a.H = pl->H;
+            a.h = h;
+            a.nsteps = cs;
+            a.off = pl->off[h];
+            a.W = W;
+            a.out = out;
+            a.out_half = out_half;
+            a.prec = pl->prec;
+            return a;
+        };
         for (int h = 1; h < pl->H; ++h) {
             TimedLaunch tl(pl, FDAS_K_CORR_SUB);
-            FDAS_TRY(launch_corr_sub(pl->L[h], pl->bank[h], pl->Z[h], pl->d_seg, pl->seg_stride,
-                                     pl->seg_off[h], cs, pl->off[h], pl->W[h],
-                                     pl->d_sub + pl->hs.sub_off[h], pl->stream));
+            void* out = static_cast<char*>(pl->d_sub) + (size_t)pl->hs.sub_off[h] * pl->sub_bytes;
+            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, out, pl->W[h], pl->sub_bytes == 2),
+                                 pl->stream));
         }
         float* plane = d_planes ? d_planes + (size_t)(c0 - step0) * pl->Z0 * U : pl->d_plane;
         if (!pl->fused_harmsum) {
             {
                 TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-                FDAS_TRY(launch_corr_sub(pl->L[0], pl->bank[0], pl->Z0, pl->d_seg, pl->seg_stride, 0,
-                                         cs, pl->off[0], U, plane, pl->stream));
+                FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U, 0), pl->stream));
             }
             TimedLaunch tl(pl, FDAS_K_HARMSUM);
             FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo,
@@ -924,8 +986,9 @@ extern "C" int fdas_run_steps(fdas_plan* pl, int step0, int nsteps, float* d_pla
             continue;
         }
         TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-        FDAS_TRY(launch_corr_fund(pl->L[0], pl->bank[0], pl->Z0, pl->d_seg, pl->seg_stride, c0, cs,
-                                  pl->off[0], pl->d_fullrlo, pl->d_rlo_h, pl->d_sub, pl->hs, plane,
+        FDAS_TRY(launch_corr_fund(pl->L[0], pl->bank[0], pl->Z0, static_cast<const cplx*>(pl->d_seg),
+                                  pl->seg_stride, c0, cs, pl->off[0], pl->d_fullrlo, pl->d_rlo_h,
+                                  static_cast<const float*>(pl->d_sub), pl->hs, plane,
                                   pl->d_cands, pl->d_ncand, pl->stream));
     }
     return FDAS_OK;

@@ -97,6 +97,8 @@ struct SegDesc {       // one (step, harmonic) segment of the overlap-save loop
     int lobin;         // first spectrum bin fetched
     int numdata;       // bins normalised (median taken over these)
     long long out;     // element offset of its L/2 transformed amplitudes
+    int eslot;         // step * H + harmonic: slot of its storage exponent
+    int pad;
 };
 
 struct HarmSumArgs {
@@ -112,18 +114,49 @@ struct HarmSumArgs {
     long long sub_stride[MAX_HARM];  // elements per step (Z*W)
     float powcut[5];
     int cand_cap;
+    int sub_half;                 // sub-harmonic planes stored as FP16
 };
 
 // Normalise + transform every segment in desc[0..nseg) (all of FFT length L);
-// segment d lands at d_seg + (d.out - out_base).
-int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_desc, int nseg,
-                        cplx* d_seg, long long out_base, hipStream_t s);
+// segment d lands at seg + (d.out - out_base) in the storage format of prec, its
+// exponent (reduced precision) at seg_exp[d.eslot - eslot_base].
+struct SegPrepArgs {
+    const cplx* spec;
+    long nbins;
+    const SegDesc* desc;
+    int nseg;
+    void* seg;
+    long long out_base;
+    int* seg_exp;
+    int eslot_base;
+    int prec;
+};
+int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s);
 
-// Fused correlation + inverse FFT + power for one sub-harmonic:
-// out[(s*Z + z)*W + i] = |IFFT(A_s * conj(K_z))[off + i]|^2 / L^2, i < W.
-int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
-                    long long seg_off, int nsteps, int off, int W, float* d_out,
-                    hipStream_t s);
+// Fused correlation + inverse FFT + power for one harmonic over a batch of steps:
+// out[(s*Z + z)*W + i] = |IFFT(A_s * conj(K_z))[off + i]|^2 / L^2, i < W, with
+// A_s at seg + s*seg_stride + seg_off.  Reduced precision: bank/seg packed, with
+// exponents bank_exp[z] and seg_exp[s*H + h]; out is FP16 when out_half.
+struct CorrArgs {
+    const void* bank;
+    const int* bank_exp;
+    int Z;
+    const void* seg;
+    long long seg_stride;
+    long long seg_off;
+    const int* seg_exp;
+    int H, h;
+    int nsteps;
+    int off;
+    int W;
+    void* out;
+    int out_half;
+    int prec;
+};
+int launch_corr(int L, const CorrArgs& a, hipStream_t s);
+// Reduced-precision copy of a bank of Z rows of length L.
+int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
+                     hipStream_t s);
 
 // Fundamental version with the harmonic sum, stage thresholds and plane write.
 int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
@@ -133,7 +166,7 @@ int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long l
 
 // Harmonic sum in place over d_plane[step][Z0][U] (fundamental powers on entry),
 // stage thresholds and candidate emission.
-int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s);
 

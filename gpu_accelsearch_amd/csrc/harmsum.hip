@@ -15,6 +15,7 @@
 #include <cstdlib>
 
 #include "fdas_internal.hpp"
+#include "precision.hpp"
 
 namespace fdas {
 
@@ -56,9 +57,9 @@ __device__ __forceinline__ void hs_emit(fdas_candidate* cands, int* ncand, int c
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
 // fundamental powers on entry and the summed powers on exit.
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename SubE>
 __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
-                                                       const float* __restrict__ sub,
+                                                       const SubE* __restrict__ sub,
                                                        HarmSumArgs hs, int step0, int Z0,
                                                        const int* __restrict__ fullrlo,
                                                        const int* __restrict__ rlo_h,
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
             const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
             const int ri0 = hs_rind(q0 + c0, c, lm, r2);
             const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0 + 1;
-            const float* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
+            const SubE* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
                                (long long)zi0 * hs.W[g] + ri0;
             float v[HS_MAXR][HS_MAXK];
 #pragma unroll
@@ -132,7 +133,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
 #pragma unroll
                 for (int k = 0; k < HS_MAXK; ++k) {
                     const int col = t + k * HS_T;
-                    v[rr][k] = (rr < nr && col < nc) ? src[(long long)rr * hs.W[g] + col] : 0.f;
+                    v[rr][k] = (rr < nr && col < nc) ? PStore<SubE>::get(src[(long long)rr * hs.W[g] + col]) : 0.f;
                 }
             if (first) {
                 mark_fundamental();
@@ -209,7 +210,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                 const int c = hs.c[h], lm = hs.lm[h];
                 const int rind = hs_rind(q0 + col, c, lm, 2 * rl[h]);
                 const int zind = hs_zind(zz, c, lm, hs.zlo[h]);
-                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+                a2 += PStore<SubE>::get(sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind]);
                 if (((h + 1) & h) == 0) {
                     ++st;
                     if (a2 > hs.powcut[st]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, st, a2);
@@ -219,33 +220,32 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     }
 }
 
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
-static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0,
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename SubE>
+static int harmsum_t(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0,
                      int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
                      fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
 {
     dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
               (unsigned)nsteps);
-    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS>), grid, dim3(HS_T), 0, s, d_plane,
-                       d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand);
+    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS, SubE>), grid, dim3(HS_T), 0, s,
+                       d_plane, static_cast<const SubE*>(d_sub), hs, step0, Z0, d_fullrlo, d_rlo_h,
+                       d_cands, d_ncand);
     return check_launch("harmsum_kernel");
 }
 
-int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s)
 {
     if (nsteps <= 0) return FDAS_OK;
     // Tile shape: 2 rows x 2048 columns, 256 threads, 32 KB of LDS measured
     // fastest of the shapes tried (long contiguous sub-harmonic runs matter more
-    // than the staged volume).  FDAS_HS_TILE=1 selects 8 x 512.
-    static const int tile = [] {
-        const char* e = getenv("FDAS_HS_TILE");
-        return e ? atoi(e) : 0;
-    }();
-    if (tile == 1)
-        return harmsum_t<256, 8, 512, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
-    return harmsum_t<256, 2, 2048, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
+    // than the staged volume).
+    if (hs.sub_half)
+        return harmsum_t<256, 2, 2048, 8192, unsigned short>(d_plane, d_sub, hs, step0, nsteps, Z0,
+                                                             d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
+    return harmsum_t<256, 2, 2048, 8192, float>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo,
+                                                d_rlo_h, d_cands, d_ncand, s);
 }
 
 }  // namespace fdas

@@ -1,0 +1,92 @@
+// precision.hpp -- storage formats of spectra, templates and power planes.
+//
+// FP32 mode stores complex values as two floats.  The reduced-precision modes
+// store a complex value as two 16-bit halves packed in 32 bits (FP16 or BF16,
+// round to nearest even) together with one power-of-two exponent per template row
+// or per segment, chosen so that the largest component maps below 2^15.  Scaling
+// by a power of two is exact, so the exponents are folded into the power scale
+// (|y|^2 scales with 2^(2 (e_template + e_segment))).  All FFT arithmetic and all
+// power accumulation stay in FP32.  PStore<unsigned short> (FP16 power planes,
+// saturating at 65504) is available to the correlation and harmonic-sum kernels.
+#pragma once
+
+#include <hip/hip_fp16.h>
+
+#include "fdas_common.hpp"
+
+namespace fdas {
+
+enum Prec : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
+
+__device__ __forceinline__ unsigned short f32_to_f16(float x) { return __half_as_ushort(__float2half_rn(x)); }
+__device__ __forceinline__ float f16_to_f32(unsigned short b) { return __half2float(__ushort_as_half(b)); }
+__device__ __forceinline__ unsigned short f32_to_bf16(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);   // round to nearest even (finite inputs)
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(unsigned short b) { return __uint_as_float(((unsigned)b) << 16); }
+
+template <int P>
+struct CStore;
+
+template <>
+struct CStore<PREC_F32> {
+    using T = cplx;
+    __device__ __forceinline__ static cplx get(T v) { return v; }
+    __device__ __forceinline__ static T put(cplx v) { return v; }
+};
+
+template <>
+struct CStore<PREC_F16> {
+    using T = unsigned;
+    __device__ __forceinline__ static cplx get(T v)
+    {
+        return mk(f16_to_f32((unsigned short)(v & 0xffffu)), f16_to_f32((unsigned short)(v >> 16)));
+    }
+    __device__ __forceinline__ static T put(cplx v)
+    {
+        return (unsigned)f32_to_f16(v.x) | ((unsigned)f32_to_f16(v.y) << 16);
+    }
+};
+
+template <>
+struct CStore<PREC_BF16> {
+    using T = unsigned;
+    __device__ __forceinline__ static cplx get(T v)
+    {
+        return mk(bf16_to_f32((unsigned short)(v & 0xffffu)), bf16_to_f32((unsigned short)(v >> 16)));
+    }
+    __device__ __forceinline__ static T put(cplx v)
+    {
+        return (unsigned)f32_to_bf16(v.x) | ((unsigned)f32_to_bf16(v.y) << 16);
+    }
+};
+
+// Power-plane element: float, or FP16 (saturating at the largest finite value).
+template <typename E>
+struct PStore;
+template <>
+struct PStore<float> {
+    __device__ __forceinline__ static float get(float v) { return v; }
+    __device__ __forceinline__ static float put(float v) { return v; }
+};
+template <>
+struct PStore<unsigned short> {
+    __device__ __forceinline__ static float get(unsigned short v) { return f16_to_f32(v); }
+    __device__ __forceinline__ static unsigned short put(float v) { return f32_to_f16(fminf(v, 65504.f)); }
+};
+
+// Exponent e with max_abs * 2^-e < 2^15 (0 for an all-zero block).
+__device__ __forceinline__ int block_exponent(float max_abs)
+{
+    if (!(max_abs > 0.f)) return 0;
+    int e2;
+    (void)frexpf(max_abs, &e2);   // max_abs = m 2^e2, m in [0.5, 1)
+    return e2 - 15;
+}
+
+inline size_t cstore_bytes(int prec) { return prec == PREC_F32 ? sizeof(cplx) : 4; }
+
+}  // namespace fdas

@@ -22,6 +22,7 @@
 
 #include "fdas_internal.hpp"
 #include "fft_block.hpp"
+#include "precision.hpp"
 
 namespace fdas {
 
@@ -103,16 +104,38 @@ __device__ __forceinline__ void select_digit(unsigned* hist, int* sel, int k, in
 }
 
 // ------------------------------------------------------------------ fused path
-template <int NH>
+// Largest |re| or |im| over the workgroup's values (all threads get the result).
+template <int T, int E>
+__device__ __forceinline__ float block_max_abs(const cplx* v, float* red, int t)
+{
+    float m = 0.f;
+#pragma unroll
+    for (int i = 0; i < E; ++i) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) {
+        float r = red[0];
+        for (int w = 1; w < T / 64; ++w) r = fmaxf(r, red[w]);
+        red[T / 64] = r;
+    }
+    __syncthreads();
+    return red[T / 64];
+}
+
+template <int NH, int P>
 __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     const cplx* __restrict__ spec, long nbins, const SegDesc* __restrict__ desc,
-    cplx* __restrict__ seg, long long out_base, const cplx* __restrict__ tw)
+    typename CStore<P>::T* __restrict__ seg, long long out_base, int* __restrict__ seg_exp,
+    int eslot_base, const cplx* __restrict__ tw)
 {
     constexpr int T = FftCfgS<NH>::T;
     using F = BlockFFT<NH, T, -1>;
     __shared__ cplx lds[F::LDS_ELEMS];
     __shared__ unsigned hist[256];
     __shared__ int sel[2];
+    __shared__ float red[T / 64 + 1];
     const SegDesc d = desc[blockIdx.x];
     const int t = threadIdx.x;
     cplx v[F::E];
@@ -147,15 +170,24 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     for (int i = 0; i < F::E; ++i)
         if (F::in_index(t, i) < d.numdata) v[i] = scale_nc(v[i], c);
     F::run(v, lds, tw, t);
-    cplx* o = seg + (d.out - out_base);
+    typename CStore<P>::T* o = seg + (d.out - out_base);
+    if constexpr (P == PREC_F32) {
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) o[F::out_index(t, i)] = v[i];
+        for (int i = 0; i < F::E; ++i) o[F::out_index(t, i)] = v[i];
+    } else {
+        const int e = block_exponent(block_max_abs<T, F::E>(v, red, t));
+        if (t == 0) seg_exp[d.eslot - eslot_base] = e;
+#pragma unroll
+        for (int i = 0; i < F::E; ++i)
+            o[F::out_index(t, i)] = CStore<P>::put(mk(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e)));
+    }
 }
 
-// Template values of row z at this thread's first-pass positions, kept in
-// registers while the workgroup walks its group of steps.
-template <int L, int T = FftCfgS<L>::T>
-__device__ __forceinline__ void load_template(const cplx* __restrict__ K, cplx* kr, int t)
+// Template values of row z at this thread's first-pass positions (stored form),
+// kept in registers while the workgroup walks its group of steps.
+template <int L, int T = FftCfgS<L>::T, int P = PREC_F32>
+__device__ __forceinline__ void load_template(const typename CStore<P>::T* __restrict__ K,
+                                              typename CStore<P>::T* kr, int t)
 {
     using F = BlockFFT<L, T, 1>;
 #pragma unroll
@@ -164,52 +196,83 @@ __device__ __forceinline__ void load_template(const cplx* __restrict__ K, cplx* 
 
 // y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.  Position t + i*T needs
 // A[t + (i mod E/2)*T], so each thread loads E/2 segment values.
-template <int L, int T = FftCfgS<L>::T>
-__device__ __forceinline__ void corr_ifft(const cplx* kr, const cplx* __restrict__ A,
+template <int L, int T = FftCfgS<L>::T, int P = PREC_F32>
+__device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
+                                          const typename CStore<P>::T* __restrict__ A,
                                           const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
 {
     using F = BlockFFT<L, T, 1>;
     constexpr int EH = F::E / 2;
     cplx a[EH];
 #pragma unroll
-    for (int i = 0; i < EH; ++i) a[i] = A[t + i * T];
+    for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[t + i * T]);
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) v[i] = cmulc(a[i % EH], kr[i]);
+    for (int i = 0; i < F::E; ++i) v[i] = cmulc(a[i % EH], CStore<P>::get(kr[i]));
     F::run(v, lds, tw, t);
 }
 
 // Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps))
-// with the template row held in registers across them.
-template <int L, int G, int T>
+// with the template row held in registers across them.  Reduced-precision
+// inputs carry per-row / per-segment exponents, folded into the power scale.
+template <int L, int G, int T, int P, typename OutE>
 __global__ __launch_bounds__(T) void corr_sub_kernel(
-    const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    long long seg_off, int nsteps, int off, int W, float* __restrict__ out,
-    const cplx* __restrict__ tw, float scale)
+    const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
+    const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
+    const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
+    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale)
 {
     using F = BlockFFT<L, T, 1>;
+    using S = typename CStore<P>::T;
     __shared__ cplx lds[F::LDS_ELEMS];
     const int z = blockIdx.x, t = threadIdx.x;
     const int s0 = blockIdx.y * G;
     // G == 1: exactly one step (a loop would let the compiler hoist every twiddle)
     const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
     cplx v[F::E];
-    [[maybe_unused]] cplx kr[G > 1 ? F::E : 1];
-    if constexpr (G > 1) load_template<L, T>(bank + (size_t)z * L, kr, t);
+    [[maybe_unused]] S kr[G > 1 ? F::E : 1];
+    if constexpr (G > 1) load_template<L, T, P>(bank + (size_t)z * L, kr, t);
+    const int ek = P == PREC_F32 ? 0 : bank_exp[z];
     for (int s = s0; s < s1; ++s) {
         if constexpr (G > 1) {
-            corr_ifft<L, T>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T, P>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
         } else {
-            cplx k1[F::E];
-            load_template<L, T>(bank + (size_t)z * L, k1, t);
-            corr_ifft<L, T>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            S k1[F::E];
+            load_template<L, T, P>(bank + (size_t)z * L, k1, t);
+            corr_ifft<L, T, P>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
         }
-        float* o = out + ((size_t)s * Z + z) * W;
+        const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
+        OutE* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
         for (int i = 0; i < F::E; ++i) {
             const int col = F::out_index(t, i) - off;
-            if (col >= 0 && col < W) o[col] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
+            if (col >= 0 && col < W)
+                o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
         }
     }
+}
+
+// Reduced-precision copy of a template bank: per row the exponent and the packed
+// values (one workgroup per row).
+template <int P>
+__global__ __launch_bounds__(256) void bank_pack_kernel(const cplx* __restrict__ in, int L,
+                                                        typename CStore<P>::T* __restrict__ out,
+                                                        int* __restrict__ row_exp)
+{
+    __shared__ float red[5];
+    const int z = blockIdx.x, t = threadIdx.x;
+    const cplx* row = in + (size_t)z * L;
+    float m = 0.f;
+    for (int i = t; i < L; i += 256) m = fmaxf(m, fmaxf(fabsf(row[i].x), fabsf(row[i].y)));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((t & 63) == 0) red[t >> 6] = m;
+    __syncthreads();
+    if (t == 0) red[4] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    __syncthreads();
+    const int e = block_exponent(red[4]);
+    if (t == 0) row_exp[z] = e;
+    for (int i = t; i < L; i += 256)
+        out[(size_t)z * L + i] = CStore<P>::put(mk(ldexpf(row[i].x, -e), ldexpf(row[i].y, -e)));
 }
 
 __device__ __forceinline__ void emit_candidate(fdas_candidate* cands, int* ncand, int cap,
@@ -386,9 +449,8 @@ __global__ __launch_bounds__(FftCfgS<L>::T) void corr_fund_kernel(
 }
 
 // ------------------------------------------------------------------ launchers
-template <int NH>
-static int seg_prep_t(const cplx* spec, long nbins, const SegDesc* desc, int nseg, cplx* seg,
-                      long long out_base, hipStream_t s)
+template <int NH, int P>
+static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
 {
     int err = FDAS_OK;
     const cplx* tw = fft_twiddles(NH, &err);
@@ -397,76 +459,113 @@ static int seg_prep_t(const cplx* spec, long nbins, const SegDesc* desc, int nse
         set_error("thread count mismatch for length %d", NH);
         return FDAS_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(seg_prep_kernel<NH>, dim3(nseg), dim3(FftCfgS<NH>::T), 0, s, spec, nbins,
-                       desc, seg, out_base, tw);
+    hipLaunchKernelGGL((seg_prep_kernel<NH, P>), dim3(a.nseg), dim3(FftCfgS<NH>::T), 0, s, a.spec,
+                       a.nbins, a.desc, static_cast<typename CStore<P>::T*>(a.seg), a.out_base,
+                       a.seg_exp, a.eslot_base, tw);
     return check_launch("seg_prep_kernel");
 }
 
-int launch_segment_prep(int L, const cplx* d_spec, long nbins, const SegDesc* d_desc, int nseg,
-                        cplx* d_seg, long long out_base, hipStream_t s)
+template <int NH>
+static int seg_prep_p(const SegPrepArgs& a, hipStream_t s)
 {
-    if (nseg <= 0) return FDAS_OK;
+    switch (a.prec) {
+        case PREC_F16: return seg_prep_t<NH, PREC_F16>(a, s);
+        case PREC_BF16: return seg_prep_t<NH, PREC_BF16>(a, s);
+        default: return seg_prep_t<NH, PREC_F32>(a, s);
+    }
+}
+
+int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
+{
+    if (a.nseg <= 0) return FDAS_OK;
     switch (L / 2) {
-        case 128: return seg_prep_t<128>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 256: return seg_prep_t<256>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 512: return seg_prep_t<512>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 1024: return seg_prep_t<1024>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 2048: return seg_prep_t<2048>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 4096: return seg_prep_t<4096>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
-        case 8192: return seg_prep_t<8192>(d_spec, nbins, d_desc, nseg, d_seg, out_base, s);
+        case 128: return seg_prep_p<128>(a, s);
+        case 256: return seg_prep_p<256>(a, s);
+        case 512: return seg_prep_p<512>(a, s);
+        case 1024: return seg_prep_p<1024>(a, s);
+        case 2048: return seg_prep_p<2048>(a, s);
+        case 4096: return seg_prep_p<4096>(a, s);
+        case 8192: return seg_prep_p<8192>(a, s);
         default:
             set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
             return FDAS_ERR_UNSUPPORTED;
     }
 }
 
-template <int L, int T = FftCfgS<L>::T>
-static int corr_sub_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride,
-                      long long seg_off, int nsteps, int off, int W, float* out, hipStream_t s)
+// Threads of the fused correlation at each length: 512 (16 values per thread) at
+// 4096 and 8192 measured faster than 256 (more waves per SIMD outweigh the extra
+// LDS pass).
+template <int L>
+struct CorrT {
+    static constexpr int T = (L == 4096 || L == 8192) ? 512 : FftCfgS<L>::T;
+};
+
+template <int L, int P, typename OutE>
+static int corr_t(const CorrArgs& a, hipStream_t s)
 {
+    constexpr int T = CorrT<L>::T;
+    // Templates stay in registers across 8 steps up to length 8192; at 16384 the
+    // registers are needed for the transform itself.
+    constexpr int G = L <= 8192 ? 8 : 1;
+    using S = typename CStore<P>::T;
     int err = FDAS_OK;
     const cplx* tw = fft_twiddles_t(L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    // Templates stay in registers across 8 steps up to length 8192; at 16384 the
-    // registers are needed for the transform itself.
-    constexpr int G = L <= 8192 ? 8 : 1;
-    hipLaunchKernelGGL((corr_sub_kernel<L, G, T>), dim3(Z, (nsteps + G - 1) / G), dim3(T), 0, s,
-                       bank, Z, seg, seg_stride, seg_off, nsteps, off, W, out, tw, scale);
+    hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), dim3(a.Z, (a.nsteps + G - 1) / G),
+                       dim3(T), 0, s, static_cast<const S*>(a.bank), a.bank_exp, a.Z,
+                       static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h,
+                       a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
     return check_launch("corr_sub_kernel");
 }
 
-// Threads of the fused correlation at each length: 512 (16 values per thread) at
-// 4096 and 8192 measured faster than 256 (more waves per SIMD outweigh the extra
-// LDS pass).  FDAS_CORR8192_T / FDAS_CORR4096_T=256 select the 256-thread form.
-static bool corr_t256(const char* var)
+template <int L>
+static int corr_l(const CorrArgs& a, hipStream_t s)
 {
-    const char* e = getenv(var);
-    return e && atoi(e) == 256;
+    switch (a.prec) {
+        case PREC_F16:
+            return a.out_half ? corr_t<L, PREC_F16, unsigned short>(a, s)
+                              : corr_t<L, PREC_F16, float>(a, s);
+        case PREC_BF16: return corr_t<L, PREC_BF16, float>(a, s);
+        default: return corr_t<L, PREC_F32, float>(a, s);
+    }
 }
 
-int launch_corr_sub(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
-                    long long seg_off, int nsteps, int off, int W, float* d_out, hipStream_t s)
+int launch_corr(int L, const CorrArgs& a, hipStream_t s)
 {
-    if (nsteps <= 0) return FDAS_OK;
+    if (a.nsteps <= 0) return FDAS_OK;
+    if (a.out_half && a.prec != PREC_F16) {
+        set_error("FP16 power planes need FP16 inputs");
+        return FDAS_ERR_INVALID;
+    }
     switch (L) {
-        case 256: return corr_sub_t<256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 512: return corr_sub_t<512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 1024: return corr_sub_t<1024>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 2048: return corr_sub_t<2048>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 4096:
-            if (corr_t256("FDAS_CORR4096_T"))
-                return corr_sub_t<4096, 256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-            return corr_sub_t<4096, 512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 8192:
-            if (corr_t256("FDAS_CORR8192_T"))
-                return corr_sub_t<8192, 256>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-            return corr_sub_t<8192, 512>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
-        case 16384: return corr_sub_t<16384>(d_bank, Z, d_seg, seg_stride, seg_off, nsteps, off, W, d_out, s);
+        case 256: return corr_l<256>(a, s);
+        case 512: return corr_l<512>(a, s);
+        case 1024: return corr_l<1024>(a, s);
+        case 2048: return corr_l<2048>(a, s);
+        case 4096: return corr_l<4096>(a, s);
+        case 8192: return corr_l<8192>(a, s);
+        case 16384: return corr_l<16384>(a, s);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;
     }
+}
+
+int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
+                     hipStream_t s)
+{
+    if (prec == PREC_F16)
+        hipLaunchKernelGGL(bank_pack_kernel<PREC_F16>, dim3(Z), dim3(256), 0, s, d_bank, L,
+                           static_cast<unsigned*>(d_out), d_row_exp);
+    else if (prec == PREC_BF16)
+        hipLaunchKernelGGL(bank_pack_kernel<PREC_BF16>, dim3(Z), dim3(256), 0, s, d_bank, L,
+                           static_cast<unsigned*>(d_out), d_row_exp);
+    else {
+        set_error("bank packing needs a reduced-precision mode");
+        return FDAS_ERR_INVALID;
+    }
+    return check_launch("bank_pack_kernel");
 }
 
 template <int L>

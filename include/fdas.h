@@ -141,6 +141,9 @@ enum {
     FDAS_ERR_UNSUPPORTED = 4  /* shape outside what the kernels implement    */
 };
 
+/* Storage of spectra segments and templates (FFT arithmetic, power planes and
+ * power accumulation are FP32 in every mode): FP32, FP16 or BF16.  Reduced-
+ * precision values carry power-of-two exponents per template row and segment. */
 enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2 };
 
 #define FDAS_MAX_HARM 16

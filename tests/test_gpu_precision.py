@@ -1,0 +1,83 @@
+"""Reduced-precision storage (FP16 / BF16 spectra segments and templates, FP16
+sub-harmonic power planes in FP16 mode) against the FP64 oracle.
+
+Reported per mode: the largest |dP| / max(P_ref, 1) over the checked steps, and
+agreement of the candidate sets wherever the oracle's stage sums are farther
+than that error from the threshold."""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+# measured bounds (precision section)
+TOL = {"fp16": 2e-3, "bf16": 2e-2}
+
+
+def power_err(p, pref):
+    return float(np.max(np.abs(p.astype(np.float64) - pref) / np.maximum(pref, 1.0)))
+
+
+@pytest.fixture(scope="module")
+def fd():
+    import gpu_accelsearch_amd as fd
+    fd.lib()
+    return fd
+
+
+@pytest.mark.parametrize("mode", ["fp16", "bf16"])
+def test_precision_plain(fd, oracle, mode):
+    from gpu_accelsearch_amd import synth
+    n = 1 << 20
+    x = synth.plain_series(n, seed=21, r0=44000.7, z0=12.0, power=90.0)
+    print_case("precision plain", mode)
+    P = fd.SearchParams(n=n, zmax=50, numharm=4, fftlen=2048, precision=mode)
+    with fd.AccelSearch(P) as s:
+        ref = oracle.Search(x, n, 6.4e-5, 50, 4, 2048)
+        s.prepare(x)
+        worst = 0.0
+        for st in (0, 40, synth.steps_covering(4 * 44000.7, s.rlo, s.uselen), s.nsteps - 1):
+            worst = max(worst, power_err(s.run_steps(st, 1)[0], ref.step_plane(st)))
+        print(f"{mode} max relative power error {worst:.3e}")
+        assert worst <= TOL[mode], worst
+        ref.close()
+
+
+@pytest.mark.parametrize("mode", ["fp16", "bf16"])
+def test_precision_production_candidates(fd, oracle, mode):
+    """2^23 samples, zmax 200, 8 harmonics: powers on the pulsar steps and the
+    candidate sets there."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 23
+    x = synth.production_series(n)
+    powcut = (40.0, 45.0, 55.0, 70.0, 0.0)
+    print_case("precision production", mode)
+    P = fd.SearchParams(n=n, zmax=200, numharm=8, fftlen=8192, precision=mode, powcut=powcut)
+    with fd.AccelSearch(P) as s:
+        ref = oracle.Search(x, n, 6.4e-5, 200, 8, 8192)
+        steps = sorted({synth.steps_covering(8 * r0, s.rlo, s.uselen)
+                        for r0, _, _ in synth.PRODUCTION_PULSARS})
+        s.prepare(x)
+        worst = 0.0
+        for st in steps:
+            worst = max(worst, power_err(s.run_steps(st, 1)[0], ref.step_plane(st)))
+        print(f"{mode} production max relative power error {worst:.3e}")
+        assert worst <= TOL[mode], worst
+        s.search(x)
+        got = {(int(c["step"]), int(c["z_index"]), int(c["r_index"]), int(c["stage"]))
+               for c in s.candidates() if int(c["step"]) in steps}
+        want, ambiguous = set(), set()
+        for st in steps:
+            _, stages = ref.step_plane(st, stages=True)
+            for sg in range(4):
+                z, i = np.nonzero(stages[sg] > powcut[sg])
+                want |= {(st, int(a), int(b), sg) for a, b in zip(z, i)}
+                z, i = np.nonzero(np.abs(stages[sg] - powcut[sg]) <=
+                                  TOL[mode] * np.maximum(stages[sg], 1.0))
+                ambiguous |= {(st, int(a), int(b), sg) for a, b in zip(z, i)}
+        assert len(want) > 0
+        assert got - ambiguous == want - ambiguous
+        print(f"{mode}: {len(want)} oracle candidates, {len(got)} found, "
+              f"{len(ambiguous)} within the error of the threshold")
+        ref.close()
