@@ -50,7 +50,13 @@ class FdasPlanInfo(C.Structure):
                 ("fftlen", C.c_int * MAX_HARM), ("numzs", C.c_int * MAX_HARM),
                 ("harm_cur", C.c_int * MAX_HARM), ("harm_max", C.c_int * MAX_HARM),
                 ("chunk_steps", C.c_int), ("nbins", C.c_long), ("cells", C.c_longlong),
-                ("sub_cells", C.c_longlong * MAX_HARM), ("device_bytes", C.c_size_t)]
+                ("sub_cells", C.c_longlong * MAX_HARM), ("device_bytes", C.c_size_t),
+                ("ntrials", C.c_int), ("template_bytes", C.c_size_t)]
+
+
+class FdasTrialInfo(C.Structure):
+    _fields_ = [("n", C.c_long), ("nbins", C.c_long), ("rlo", C.c_int), ("nsteps", C.c_int),
+                ("first_step", C.c_int)]
 
 
 class FdasTiming(C.Structure):
@@ -62,7 +68,7 @@ KERNEL_CATEGORIES = ["r2c", "whiten", "seg_prep", "corr_sub", "corr_fund", "harm
 
 class FdasCandidate(C.Structure):
     _fields_ = [("step", C.c_int), ("z_index", C.c_int), ("r_index", C.c_int),
-                ("stage", C.c_int), ("power", C.c_float), ("reserved", C.c_float)]
+                ("stage", C.c_int), ("power", C.c_float), ("trial", C.c_int)]
 
 
 vp = C.c_void_p
@@ -114,7 +120,15 @@ PROTOTYPES = [
     ("fdas_get_spectrum", i32, [vp, vp]),
     ("fdas_get_candidates", i32, [vp, C.POINTER(FdasCandidate), i32, C.POINTER(i32),
                                   C.POINTER(i32)]),
+    ("fdas_reset_candidates", i32, [vp]),
     ("fdas_synchronize", i32, [vp]),
+    ("fdas_batch_create", i32, [C.POINTER(FdasParams), i32, PLong, C.POINTER(vp)]),
+    ("fdas_batch_trial_info", i32, [vp, i32, C.POINTER(FdasTrialInfo)]),
+    ("fdas_batch_series", C.POINTER(C.c_float), [vp, i32]),
+    ("fdas_prepare_trial", i32, [vp, i32, vp]),
+    ("fdas_batch_run", i32, [vp]),
+    ("fdas_plan_enable_graph", i32, [vp, i32]),
+    ("fdas_batch_spectrum", i32, [vp, i32, vp]),
     ("fdas_plan_enable_timing", i32, [vp, i32]),
     ("fdas_plan_get_timing", i32, [vp, C.POINTER(FdasTiming)]),
     ("fdas_malloc", i32, [C.POINTER(vp), C.c_size_t]),

@@ -95,7 +95,7 @@ class SearchParams:
 
 
 CANDIDATE_DTYPE = np.dtype([("step", np.int32), ("z_index", np.int32), ("r_index", np.int32),
-                            ("stage", np.int32), ("power", np.float32), ("reserved", np.float32)])
+                            ("stage", np.int32), ("power", np.float32), ("trial", np.int32)])
 
 
 class AccelSearch:
@@ -195,6 +195,104 @@ class AccelSearch:
             self.close()
         except Exception:
             pass
+
+
+class BatchSearch:
+    """Many DM trials (any even lengths, one zmax/numharm/fftlen) in one plan.
+
+    The reference runs AccelSearch.cu's loop once per series; here every
+    (trial, step) pair is an item of one chunked launch sequence, the plan owns
+    one device input buffer per trial, and with ``graph=True`` the whole batch is
+    recorded once into a hipGraph and replayed.  Template banks are shared by all
+    live plans of the same configuration."""
+
+    def __init__(self, params: SearchParams, lengths, graph: bool = False):
+        self.params = params
+        self.lengths = [int(n) for n in lengths]
+        arr = (C.c_long * len(self.lengths))(*self.lengths)
+        h = C.c_void_p()
+        cp = params.to_c()
+        check(lib().fdas_batch_create(C.byref(cp), len(self.lengths), arr, C.byref(h)),
+              "fdas_batch_create")
+        self.handle = h.value
+        info = N.FdasPlanInfo()
+        check(lib().fdas_plan_get_info(self.handle, C.byref(info)), "fdas_plan_get_info")
+        self.info = info
+        self.uselen, self.numz, self.cells = info.uselen, info.numz, info.cells
+        self.trials = []
+        for t in range(len(self.lengths)):
+            ti = N.FdasTrialInfo()
+            check(lib().fdas_batch_trial_info(self.handle, t, C.byref(ti)), "fdas_batch_trial_info")
+            self.trials.append(ti)
+        if graph:
+            check(lib().fdas_plan_enable_graph(self.handle, 1), "fdas_plan_enable_graph")
+
+    @property
+    def ntrials(self) -> int:
+        return len(self.lengths)
+
+    def series_ptr(self, t: int) -> int:
+        p = lib().fdas_batch_series(self.handle, t)
+        if not p:
+            raise N.FdasError(N.last_error())
+        return C.cast(p, C.c_void_p).value
+
+    def upload(self, t: int, series: np.ndarray) -> None:
+        x = np.ascontiguousarray(series, dtype=np.float32)
+        if x.shape != (self.lengths[t],):
+            raise ValueError(f"trial {t}: expected ({self.lengths[t]},) floats, got {x.shape}")
+        check(lib().fdas_memcpy_htod(self.series_ptr(t), x.ctypes.data, x.nbytes), "htod")
+
+    def copy_in(self, t: int, d: DeviceArray) -> None:
+        if d.nbytes != 4 * self.lengths[t]:
+            raise ValueError(f"trial {t}: expected {4 * self.lengths[t]} bytes, got {d.nbytes}")
+        check(lib().fdas_memcpy_dtod(self.series_ptr(t), d.ptr, d.nbytes), "dtod")
+
+    def run(self) -> None:
+        check(lib().fdas_batch_run(self.handle), "fdas_batch_run")
+
+    def run_steps(self, step0: int, nsteps: int, out: DeviceArray | None = None):
+        """Plan steps [step0, step0+nsteps) (numbered over all trials) of the
+        prepared spectra; returns the summed planes unless ``out`` is given."""
+        dev = out if out is not None else DeviceArray((nsteps, self.numz, self.uselen), np.float32)
+        check(lib().fdas_run_steps(self.handle, step0, nsteps, dev.ptr), "fdas_run_steps")
+        self.synchronize()
+        return dev if out is not None else dev.to_host()
+
+    def prepare(self) -> None:
+        for t in range(self.ntrials):
+            check(lib().fdas_prepare_trial(self.handle, t, None), "fdas_prepare_trial")
+        check(lib().fdas_reset_candidates(self.handle), "fdas_reset_candidates")
+
+    def synchronize(self) -> None:
+        check(lib().fdas_synchronize(self.handle), "fdas_synchronize")
+
+    def enable_timing(self, on: bool = True) -> None:
+        check(lib().fdas_plan_enable_timing(self.handle, int(on)), "fdas_plan_enable_timing")
+
+    def timing(self) -> dict:
+        t = N.FdasTiming()
+        check(lib().fdas_plan_get_timing(self.handle, C.byref(t)), "fdas_plan_get_timing")
+        return {name: (t.launches[i], t.ms[i]) for i, name in enumerate(N.KERNEL_CATEGORIES)}
+
+    def spectrum(self, t: int) -> np.ndarray:
+        d = DeviceArray((self.trials[t].nbins,), np.complex64)
+        check(lib().fdas_batch_spectrum(self.handle, t, d.ptr), "fdas_batch_spectrum")
+        self.synchronize()
+        return d.to_host()
+
+    candidates = AccelSearch.candidates
+
+    def candidate_rz(self, c: np.ndarray):
+        rlo = np.array([tr.rlo for tr in self.trials], np.float64)[c["trial"]]
+        r = rlo + c["step"].astype(np.float64) * (self.uselen // 2) + 0.5 * c["r_index"]
+        z = -self.params.zmax + 2.0 * c["z_index"]
+        return r, z
+
+    close = AccelSearch.close
+    __enter__ = AccelSearch.__enter__
+    __exit__ = AccelSearch.__exit__
+    __del__ = AccelSearch.__del__
 
 
 def stage_powcut(sigma: float, numharm: int, numindep: float = 1.0) -> float:

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfdas.so")
-SOURCES = ["api.hip", "fft.hip", "whiten.hip", "templates.hip", "search.hip", "harmsum.hip"]
+SOURCES = ["api.hip", "plan.hip", "fft.hip", "whiten.hip", "templates.hip", "search.hip",
+           "harmsum.hip"]
 ARCH = os.environ.get("FDAS_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

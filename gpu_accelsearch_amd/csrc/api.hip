@@ -1,5 +1,5 @@
-// api.hip -- the C ABI (include/fdas.h): reference-named entry points and the
-// plan API of the fused search.
+// api.hip -- the C ABI (include/fdas.h): error state, memory plumbing and the
+// reference-named entry points.  The plan API lives in plan.hip.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -483,596 +483,4 @@ extern "C" void perform_harmonic_summing(unsigned short totalHarmonics, fdas_com
                               scale, d_powers, fullrlo, 0) != FDAS_OK)
             return;
     }
-}
-
-// ====================================================================== plan API
-struct fdas_plan {
-    fdas_params p{};
-    int uselen = 0, rlo = 0, nsteps = 0, H = 0, Z0 = 0, chunk = 0;
-    long nbins = 0;
-    int cur[MAX_HARM]{}, mx[MAX_HARM]{}, L[MAX_HARM]{}, Z[MAX_HARM]{}, zmax_h[MAX_HARM]{},
-        off[MAX_HARM]{}, W[MAX_HARM]{}, zlo[MAX_HARM]{};
-    cplx* bank[MAX_HARM]{};      // FP32 banks (freed after packing in reduced precision)
-    void* bankq[MAX_HARM]{};     // bank in the storage format used by the kernels
-    int* bank_exp[MAX_HARM]{};   // per-row exponents (reduced precision)
-    int prec = FDAS_PREC_FP32;
-    size_t cbytes = sizeof(cplx);   // bytes per stored complex value
-    long long seg_off[MAX_HARM]{};
-    long long seg_stride = 0;
-    long long sub_total = 0;     // elements per step over all sub-harmonics
-    long long sub_cells[MAX_HARM]{};  // sum over steps of Z_h * numrs_h
-    std::vector<int> classes;    // distinct FFT lengths
-    std::vector<std::vector<int>> class_harm;
-    std::vector<SegDesc*> d_desc;  // per class: [nsteps][harmonics in class]
-    int* d_fullrlo = nullptr;
-    int* d_rlo_h = nullptr;      // [nsteps][MAX_HARM]
-    cplx* d_spec = nullptr;
-    cplx* d_r2c_work = nullptr;
-    R2CPlan r2c;
-    WhitenSchedule ws;
-    void* d_seg = nullptr;       // [chunk][seg_stride] stored complex values
-    int* d_seg_exp = nullptr;    // [chunk][H] segment exponents (reduced precision)
-    void* d_sub = nullptr;       // sub-harmonic planes, float or FP16
-    size_t sub_bytes = sizeof(float);
-    float* d_plane = nullptr;
-    fdas_candidate* d_cands = nullptr;
-    int* d_ncand = nullptr;
-    int cand_cap = 0;
-    HarmSumArgs hs{};
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    size_t bytes = 0;
-    // harmonic sum inside the fundamental's correlation kernel (FDAS_HARMSUM=fused)
-    // instead of the tiled harmonic-sum kernel
-    bool fused_harmsum = false;
-    // event timing
-    bool timing = false;
-    struct Rec {
-        int cat;
-        hipEvent_t a, b;
-    };
-    std::vector<Rec> recs;
-    size_t nrec = 0;
-};
-
-// Brackets one launch with events when timing is on.
-struct TimedLaunch {
-    fdas_plan* pl;
-    size_t idx = (size_t)-1;
-    TimedLaunch(fdas_plan* p, int cat) : pl(p)
-    {
-        if (!pl->timing) return;
-        if (pl->nrec == pl->recs.size()) {
-            fdas_plan::Rec r{cat, nullptr, nullptr};
-            if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
-            pl->recs.push_back(r);
-        }
-        idx = pl->nrec++;
-        pl->recs[idx].cat = cat;
-        (void)hipEventRecord(pl->recs[idx].a, pl->stream);
-    }
-    ~TimedLaunch()
-    {
-        if (idx != (size_t)-1) (void)hipEventRecord(pl->recs[idx].b, pl->stream);
-    }
-};
-
-static int plan_alloc(fdas_plan* pl, void** ptr, size_t bytes)
-{
-    FDAS_HIP(hipMalloc(ptr, bytes));
-    pl->bytes += bytes;
-    return FDAS_OK;
-}
-
-static int plan_build(fdas_plan* pl)
-{
-    const fdas_params& p = pl->p;
-    if (p.n < 256 || (p.n & 1) || p.n > (1L << 27)) {
-        set_error("series length %ld: even and in [256, 2^27] required", p.n);
-        return FDAS_ERR_UNSUPPORTED;
-    }
-    if (p.zmax < 0 || p.zmax > 1200 || (p.zmax & 1)) {
-        set_error("zmax %d must be even and in [0, 1200]", p.zmax);
-        return FDAS_ERR_INVALID;
-    }
-    if (p.numharm < 1 || p.numharm > 16 || (p.numharm & (p.numharm - 1))) {
-        set_error("numharm %d must be 1, 2, 4, 8 or 16", p.numharm);
-        return FDAS_ERR_INVALID;
-    }
-    if (p.fftlen < 1024 || p.fftlen > 16384 || (p.fftlen & (p.fftlen - 1))) {
-        set_error("fftlen %d must be a power of two in [1024, 16384]", p.fftlen);
-        return FDAS_ERR_UNSUPPORTED;
-    }
-    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_BF16) {
-        set_error("precision mode %d unknown", p.precision);
-        return FDAS_ERR_INVALID;
-    }
-    if (p.precision != FDAS_PREC_FP32 && pl->fused_harmsum) {
-        set_error("the fused harmonic-sum variant runs in FP32 only");
-        return FDAS_ERR_UNSUPPORTED;
-    }
-    pl->prec = p.precision;
-    pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : 4;
-    // power planes stay FP32: FP16 planes halve the bytes per staging load in the
-    // harmonic sum and measured slower
-    pl->sub_bytes = sizeof(float);
-    pl->uselen = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
-    if (pl->uselen <= 0 || (pl->uselen & 1) || pl->uselen >= 65536 ||
-        h_fft_length(pl->uselen, 1, 1, p.zmax) != p.fftlen) {
-        set_error("uselen %d does not give fundamental FFT length %d at zmax %d", pl->uselen,
-                  p.fftlen, p.zmax);
-        return FDAS_ERR_INVALID;
-    }
-    const int U = pl->uselen;
-    // AccelSearch.cu:101: rlo = floor(1.0 * (N * dt)) with a float product
-    const float nd = (float)p.n * p.dt;
-    pl->rlo = (int)std::floor(1.0 * nd);
-    pl->nbins = p.n / 2;
-    const double half = (double)(U / 2);
-    {
-        int ns = 0;
-        double st = pl->rlo;
-        while (st + half < (double)pl->nbins) {
-            ++ns;
-            st += half;
-        }
-        pl->nsteps = ns;
-    }
-    if (pl->nsteps <= 0) {
-        set_error("no search steps: rlo %d + uselen/2 >= %ld bins", pl->rlo, pl->nbins);
-        return FDAS_ERR_INVALID;
-    }
-    pl->H = p.numharm;
-    h_harmonic_order(pl->H, pl->cur, pl->mx);
-    long long so = 0, sub = 0;
-    for (int h = 0; h < pl->H; ++h) {
-        const double f = (double)pl->cur[h] / (double)pl->mx[h];
-        pl->zmax_h[h] = h_calc_required_z(f, p.zmax);
-        pl->Z[h] = (pl->zmax_h[h] / DZ) * 2 + 1;
-        pl->L[h] = h_fft_length(U, pl->mx[h], pl->cur[h], p.zmax);
-        pl->off[h] = NUMBETWEEN * h_z_response_halfwidth((double)-pl->zmax_h[h]);
-        pl->zlo[h] = h_calc_required_z(f, -p.zmax);
-        pl->seg_off[h] = so;
-        so += pl->L[h] / 2;
-        if (pl->L[h] < 256 || pl->L[h] > 16384) {
-            set_error("harmonic %d/%d needs FFT length %d (fused kernels: 256..16384)",
-                      pl->cur[h], pl->mx[h], pl->L[h]);
-            return FDAS_ERR_UNSUPPORTED;
-        }
-    }
-    pl->seg_stride = so;
-    pl->Z0 = pl->Z[0];
-
-    // per-step tables (source/bpd_search.cu:72-127, :139-143)
-    const int S = pl->nsteps, H = pl->H;
-    std::vector<int> fullrlo(S), rlo_h((size_t)S * MAX_HARM, 0);
-    std::vector<int> lob((size_t)S * H), nd_((size_t)S * H);
-    for (int h = 0; h < H; ++h) pl->W[h] = (h == 0) ? U : 0;
-    for (int s = 0; s < S; ++s) {
-        const double startr = pl->rlo + (double)s * half;
-        const double lastr = startr + half - DR;
-        fullrlo[s] = (int)startr;
-        for (int h = 0; h < H; ++h) {
-            const double f = (double)pl->cur[h] / (double)pl->mx[h];
-            const double drlo = h_calc_required_r(f, startr);
-            const double drhi = h_calc_required_r(f, lastr);
-            const int rl = (int)std::floor(drlo);
-            const int bo = pl->off[h] / NUMBETWEEN;
-            int nr = U;
-            if (h > 0) {
-                nr = (int)((std::ceil(drhi) - std::floor(drlo)) * RDR + DBLCORRECT) + 1;
-                if (nr % RDR) nr = (nr / RDR + 1) * RDR;
-                // indices the harmonic sum reads at the first and last column
-                const double s0 = (int)(RDR * (fullrlo[s] + 0 * DR) * f + 0.5) * DR;
-                const double s1 = (int)(RDR * (fullrlo[s] + (U - 1) * DR) * f + 0.5) * DR;
-                const int r0 = (int)((s0 - rl) * RDR), r1 = (int)((s1 - rl) * RDR);
-                if (r0 < 0 || r1 >= nr) {
-                    set_error("harmonic-sum index %d..%d outside [0,%d) at step %d", r0, r1, nr, s);
-                    return FDAS_ERR_INVALID;
-                }
-            }
-            pl->W[h] = std::max(pl->W[h], nr);
-            pl->sub_cells[h] += (long long)pl->Z[h] * nr;
-            rlo_h[(size_t)s * MAX_HARM + h] = rl;
-            const int lobin = rl - bo;
-            const int hibin = (int)std::ceil(drhi) + bo;
-            const int numdata = hibin - lobin + 1;
-            if (h_next_pow2(numdata) != pl->L[h] / 2) {
-                set_error("step %d harmonic %d/%d: %d bins to normalise do not round to fftlen/2 = %d",
-                          s, pl->cur[h], pl->mx[h], numdata, pl->L[h] / 2);
-                return FDAS_ERR_UNSUPPORTED;
-            }
-            lob[(size_t)s * H + h] = lobin;
-            nd_[(size_t)s * H + h] = numdata;
-        }
-    }
-    for (int h = 1; h < H; ++h) {
-        // rows the harmonic sum reads (monotone in the fundamental row)
-        const int c = pl->cur[h], lm = h_fast_log2(pl->mx[h]);
-        for (int j : {0, pl->Z[0] - 1}) {
-            const int zz = pl->zlo[0] + DZ * j;
-            const int zind = ((zz * c) / (1 << lm) - pl->zlo[h]) >> 1;
-            if (zind < 0 || zind >= pl->Z[h]) {
-                set_error("harmonic-sum row %d outside [0,%d) for harmonic %d", zind, pl->Z[h], h);
-                return FDAS_ERR_INVALID;
-            }
-        }
-    }
-    for (int h = 0; h < H; ++h) {
-        if (pl->off[h] + pl->W[h] > pl->L[h]) {
-            set_error("harmonic %d: %d columns from offset %d exceed FFT length %d", h, pl->W[h],
-                      pl->off[h], pl->L[h]);
-            return FDAS_ERR_INVALID;
-        }
-        if (h > 0) sub += (long long)pl->Z[h] * pl->W[h];
-    }
-    pl->sub_total = sub;
-
-    // size classes of the segment transform
-    for (int h = 0; h < H; ++h) {
-        auto it = std::find(pl->classes.begin(), pl->classes.end(), pl->L[h]);
-        if (it == pl->classes.end()) {
-            pl->classes.push_back(pl->L[h]);
-            pl->class_harm.push_back({h});
-        } else {
-            pl->class_harm[it - pl->classes.begin()].push_back(h);
-        }
-    }
-
-    // chunking
-    const size_t per_step = (size_t)pl->seg_stride * sizeof(cplx) + (size_t)sub * sizeof(float) +
-                            (size_t)pl->Z0 * U * sizeof(float);
-    int chunk = p.chunk_steps;
-    if (chunk <= 0) {
-        const size_t budget = (size_t)16 << 30;
-        chunk = (int)std::max<size_t>(1, budget / per_step);
-    }
-    pl->chunk = std::min(chunk, S);
-
-    // device buffers (everything above is host-side validation and geometry)
-    FDAS_HIP(hipStreamCreate(&pl->stream));
-    pl->own_stream = true;
-    for (int h = 0; h < H; ++h) {
-        FDAS_TRY(plan_alloc(pl, (void**)&pl->bank[h], (size_t)pl->Z[h] * pl->L[h] * sizeof(cplx)));
-        FDAS_TRY(build_template_bank(pl->bank[h], pl->zmax_h[h], pl->Z[h], pl->L[h], pl->stream));
-        if (pl->prec == FDAS_PREC_FP32) {
-            pl->bankq[h] = pl->bank[h];
-        } else {
-            FDAS_TRY(plan_alloc(pl, &pl->bankq[h], (size_t)pl->Z[h] * pl->L[h] * pl->cbytes));
-            FDAS_TRY(plan_alloc(pl, (void**)&pl->bank_exp[h], (size_t)pl->Z[h] * sizeof(int)));
-            FDAS_TRY(launch_bank_pack(pl->bank[h], pl->Z[h], pl->L[h], pl->prec, pl->bankq[h],
-                                      pl->bank_exp[h], pl->stream));
-            FDAS_HIP(hipStreamSynchronize(pl->stream));
-            FDAS_HIP(hipFree(pl->bank[h]));
-            pl->bytes -= (size_t)pl->Z[h] * pl->L[h] * sizeof(cplx);
-            pl->bank[h] = nullptr;
-        }
-    }
-    for (size_t c = 0; c < pl->classes.size(); ++c) {
-        const auto& hl = pl->class_harm[c];
-        std::vector<SegDesc> d((size_t)S * hl.size());
-        for (int s = 0; s < S; ++s)
-            for (size_t q = 0; q < hl.size(); ++q) {
-                const int h = hl[q];
-                SegDesc& e = d[(size_t)s * hl.size() + q];
-                e.lobin = lob[(size_t)s * H + h];
-                e.numdata = nd_[(size_t)s * H + h];
-                e.out = (long long)s * pl->seg_stride + pl->seg_off[h];
-                e.eslot = s * H + h;
-                e.pad = 0;
-            }
-        SegDesc* dd = nullptr;
-        FDAS_TRY(plan_alloc(pl, (void**)&dd, d.size() * sizeof(SegDesc)));
-        pl->d_desc.push_back(dd);
-        FDAS_HIP(hipMemcpy(dd, d.data(), d.size() * sizeof(SegDesc), hipMemcpyHostToDevice));
-    }
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_fullrlo, (size_t)S * sizeof(int)));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_rlo_h, rlo_h.size() * sizeof(int)));
-    FDAS_HIP(hipMemcpy(pl->d_fullrlo, fullrlo.data(), (size_t)S * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_HIP(hipMemcpy(pl->d_rlo_h, rlo_h.data(), rlo_h.size() * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_spec, (size_t)(p.n / 2 + 1) * sizeof(cplx)));
-    FDAS_TRY(r2c_plan_create(p.n, &pl->r2c, pl->stream));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_r2c_work, pl->r2c.work_elems * sizeof(cplx)));
-    FDAS_TRY(whiten_schedule_create((unsigned)pl->nbins, &pl->ws));
-    FDAS_TRY(plan_alloc(pl, &pl->d_seg, (size_t)pl->chunk * pl->seg_stride * pl->cbytes));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg_exp, (size_t)pl->chunk * H * sizeof(int)));
-    if (sub > 0)
-        FDAS_TRY(plan_alloc(pl, &pl->d_sub, (size_t)pl->chunk * sub * pl->sub_bytes));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_plane, (size_t)pl->chunk * pl->Z0 * U * sizeof(float)));
-    pl->cand_cap = p.max_candidates > 0 ? p.max_candidates : (1 << 20);
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_cands, (size_t)pl->cand_cap * sizeof(fdas_candidate)));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_ncand, sizeof(int)));
-    FDAS_HIP(hipMemsetAsync(pl->d_ncand, 0, sizeof(int), pl->stream));
-
-    // harmonic-sum arguments
-    HarmSumArgs& hs = pl->hs;
-    hs.H = H;
-    hs.U = U;
-    hs.zlo0 = pl->zlo[0];
-    long long acc = 0;
-    for (int h = 0; h < H; ++h) {
-        hs.c[h] = pl->cur[h];
-        hs.lm[h] = h_fast_log2(pl->mx[h]);
-        hs.zlo[h] = pl->zlo[h];
-        hs.Z[h] = pl->Z[h];
-        hs.W[h] = pl->W[h];
-        if (h > 0) {
-            hs.sub_off[h] = acc;
-            hs.sub_stride[h] = (long long)pl->Z[h] * pl->W[h];
-            acc += (long long)pl->chunk * hs.sub_stride[h];
-        }
-    }
-    for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
-    hs.cand_cap = pl->cand_cap;
-    hs.sub_half = pl->sub_bytes == 2;
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
-    return FDAS_OK;
-}
-
-extern "C" void fdas_plan_destroy(fdas_plan* pl)
-{
-    if (!pl) return;
-    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
-    for (int h = 0; h < MAX_HARM; ++h) {
-        if (pl->bankq[h] != pl->bank[h]) (void)hipFree(pl->bankq[h]);
-        (void)hipFree(pl->bank[h]);
-        (void)hipFree(pl->bank_exp[h]);
-    }
-    for (SegDesc* d : pl->d_desc) (void)hipFree(d);
-    (void)hipFree(pl->d_fullrlo);
-    (void)hipFree(pl->d_rlo_h);
-    (void)hipFree(pl->d_spec);
-    (void)hipFree(pl->d_r2c_work);
-    r2c_plan_destroy(&pl->r2c);
-    if (pl->ws.d_start) whiten_schedule_destroy(&pl->ws);
-    (void)hipFree(pl->d_seg);
-    (void)hipFree(pl->d_seg_exp);
-    (void)hipFree(pl->d_sub);
-    (void)hipFree(pl->d_plane);
-    (void)hipFree(pl->d_cands);
-    (void)hipFree(pl->d_ncand);
-    for (auto& r : pl->recs) {
-        (void)hipEventDestroy(r.a);
-        (void)hipEventDestroy(r.b);
-    }
-    if (pl->own_stream) (void)hipStreamDestroy(pl->stream);
-    delete pl;
-}
-
-extern "C" int fdas_plan_create(const fdas_params* params, fdas_plan** out)
-{
-    if (!params || !out) {
-        set_error("fdas_plan_create: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    *out = nullptr;
-    fdas_plan* pl = new fdas_plan();
-    pl->p = *params;
-    const char* hsenv = getenv("FDAS_HARMSUM");
-    pl->fused_harmsum = hsenv && strcmp(hsenv, "fused") == 0;
-    const int rc = plan_build(pl);
-    if (rc != FDAS_OK) {
-        fdas_plan_destroy(pl);
-        return rc;
-    }
-    *out = pl;
-    return FDAS_OK;
-}
-
-extern "C" int fdas_plan_get_info(const fdas_plan* pl, fdas_plan_info* info)
-{
-    if (!pl || !info) {
-        set_error("fdas_plan_get_info: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    memset(info, 0, sizeof(*info));
-    info->uselen = pl->uselen;
-    info->rlo = pl->rlo;
-    info->nsteps = pl->nsteps;
-    info->numharm = pl->H;
-    info->numz = pl->Z0;
-    for (int h = 0; h < pl->H; ++h) {
-        info->fftlen[h] = pl->L[h];
-        info->numzs[h] = pl->Z[h];
-        info->harm_cur[h] = pl->cur[h];
-        info->harm_max[h] = pl->mx[h];
-    }
-    info->chunk_steps = pl->chunk;
-    info->nbins = pl->nbins;
-    info->cells = (long long)pl->nsteps * pl->uselen * pl->Z0;
-    for (int h = 0; h < pl->H; ++h) info->sub_cells[h] = pl->sub_cells[h];
-    info->device_bytes = pl->bytes;
-    return FDAS_OK;
-}
-
-extern "C" int fdas_plan_set_stream(fdas_plan* pl, void* stream)
-{
-    if (!pl) {
-        set_error("fdas_plan_set_stream: null plan");
-        return FDAS_ERR_INVALID;
-    }
-    if (pl->own_stream) {
-        (void)hipStreamSynchronize(pl->stream);
-        (void)hipStreamDestroy(pl->stream);
-        pl->own_stream = false;
-    }
-    if (stream) {
-        pl->stream = (hipStream_t)stream;
-    } else {
-        FDAS_HIP(hipStreamCreate(&pl->stream));
-        pl->own_stream = true;
-    }
-    return FDAS_OK;
-}
-
-extern "C" int fdas_prepare(fdas_plan* pl, const float* d_series)
-{
-    if (!pl || !d_series) {
-        set_error("fdas_prepare: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    {
-        TimedLaunch tl(pl, FDAS_K_R2C);
-        FDAS_TRY(fft_r2c_plan(pl->r2c, d_series, pl->d_spec, pl->d_r2c_work, pl->stream));
-    }
-    {
-        TimedLaunch tl(pl, FDAS_K_WHITEN);
-        FDAS_TRY(whiten(pl->d_spec, pl->ws, pl->stream));
-    }
-    FDAS_HIP(hipMemsetAsync(pl->d_ncand, 0, sizeof(int), pl->stream));
-    return FDAS_OK;
-}
-
-extern "C" int fdas_run_steps(fdas_plan* pl, int step0, int nsteps, float* d_planes)
-{
-    if (!pl || step0 < 0 || nsteps < 0 || step0 + nsteps > pl->nsteps) {
-        set_error("fdas_run_steps: steps [%d, %d) outside [0, %d)", step0, step0 + nsteps,
-                  pl ? pl->nsteps : 0);
-        return FDAS_ERR_INVALID;
-    }
-    const int U = pl->uselen;
-    for (int c0 = step0; c0 < step0 + nsteps; c0 += pl->chunk) {
-        const int cs = std::min(pl->chunk, step0 + nsteps - c0);
-        for (size_t c = 0; c < pl->classes.size(); ++c) {
-            const int nh = (int)pl->class_harm[c].size();
-            SegPrepArgs a{};
-            a.spec = pl->d_spec;
-            a.nbins = pl->nbins;
-            a.desc = pl->d_desc[c] + (size_t)c0 * nh;
-            a.nseg = cs * nh;
-            a.seg = pl->d_seg;
-            // descriptors hold offsets as if every step had a buffer slot
-            a.out_base = (long long)c0 * pl->seg_stride;
-            a.seg_exp = pl->d_seg_exp;
-            a.eslot_base = c0 * pl->H;
-            a.prec = pl->prec;
-            TimedLaunch tl(pl, FDAS_K_SEGPREP);
-            FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
-        }
-        auto corr_args = [&](int h, void* out, int W, int out_half) {
-            CorrArgs a{};
-            a.bank = pl->bankq[h];
-            a.bank_exp = pl->bank_exp[h];
-            a.Z = pl->Z[h];
-            a.seg = pl->d_seg;
-            a.seg_stride = pl->seg_stride;
-            a.seg_off = pl->seg_off[h];
-            a.seg_exp = pl->d_seg_exp;
-            a.H = pl->H;
-            a.h = h;
-            a.nsteps = cs;
-            a.off = pl->off[h];
-            a.W = W;
-            a.out = out;
-            a.out_half = out_half;
-            a.prec = pl->prec;
-            return a;
-        };
-        for (int h = 1; h < pl->H; ++h) {
-            TimedLaunch tl(pl, FDAS_K_CORR_SUB);
-            void* out = static_cast<char*>(pl->d_sub) + (size_t)pl->hs.sub_off[h] * pl->sub_bytes;
-            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, out, pl->W[h], pl->sub_bytes == 2),
-                                 pl->stream));
-        }
-        float* plane = d_planes ? d_planes + (size_t)(c0 - step0) * pl->Z0 * U : pl->d_plane;
-        if (!pl->fused_harmsum) {
-            {
-                TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-                FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U, 0), pl->stream));
-            }
-            TimedLaunch tl(pl, FDAS_K_HARMSUM);
-            FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo,
-                                    pl->d_rlo_h, pl->d_cands, pl->d_ncand, pl->stream));
-            continue;
-        }
-        TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-        FDAS_TRY(launch_corr_fund(pl->L[0], pl->bank[0], pl->Z0, static_cast<const cplx*>(pl->d_seg),
-                                  pl->seg_stride, c0, cs, pl->off[0], pl->d_fullrlo, pl->d_rlo_h,
-                                  static_cast<const float*>(pl->d_sub), pl->hs, plane,
-                                  pl->d_cands, pl->d_ncand, pl->stream));
-    }
-    return FDAS_OK;
-}
-
-extern "C" int fdas_search(fdas_plan* pl, const float* d_series)
-{
-    FDAS_TRY(fdas_prepare(pl, d_series));
-    return fdas_run_steps(pl, 0, pl->nsteps, nullptr);
-}
-
-extern "C" int fdas_get_spectrum(fdas_plan* pl, fdas_complex* d_out)
-{
-    if (!pl || !d_out) {
-        set_error("fdas_get_spectrum: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    FDAS_HIP(hipMemcpyAsync(d_out, pl->d_spec, (size_t)pl->nbins * sizeof(cplx),
-                            hipMemcpyDeviceToDevice, pl->stream));
-    return FDAS_OK;
-}
-
-extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capacity, int* count,
-                                   int* overflowed)
-{
-    if (!pl || !count) {
-        set_error("fdas_get_candidates: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
-    int n = 0;
-    FDAS_HIP(hipMemcpy(&n, pl->d_ncand, sizeof(int), hipMemcpyDeviceToHost));
-    const int kept = std::min(n, pl->cand_cap);
-    if (overflowed) *overflowed = n > pl->cand_cap;
-    std::vector<fdas_candidate> all(kept);
-    if (kept > 0)
-        FDAS_HIP(hipMemcpy(all.data(), pl->d_cands, (size_t)kept * sizeof(fdas_candidate),
-                           hipMemcpyDeviceToHost));
-    std::sort(all.begin(), all.end(), [](const fdas_candidate& a, const fdas_candidate& b) {
-        if (a.step != b.step) return a.step < b.step;
-        if (a.z_index != b.z_index) return a.z_index < b.z_index;
-        if (a.r_index != b.r_index) return a.r_index < b.r_index;
-        return a.stage < b.stage;
-    });
-    *count = kept;
-    if (out && capacity > 0) memcpy(out, all.data(), sizeof(fdas_candidate) * std::min(kept, capacity));
-    return FDAS_OK;
-}
-
-extern "C" int fdas_synchronize(fdas_plan* pl)
-{
-    if (!pl) {
-        set_error("fdas_synchronize: null plan");
-        return FDAS_ERR_INVALID;
-    }
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
-    return FDAS_OK;
-}
-
-extern "C" int fdas_plan_enable_timing(fdas_plan* pl, int enable)
-{
-    if (!pl) {
-        set_error("fdas_plan_enable_timing: null plan");
-        return FDAS_ERR_INVALID;
-    }
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
-    pl->timing = enable != 0;
-    pl->nrec = 0;
-    return FDAS_OK;
-}
-
-extern "C" int fdas_plan_get_timing(fdas_plan* pl, fdas_timing* out)
-{
-    if (!pl || !out) {
-        set_error("fdas_plan_get_timing: null argument");
-        return FDAS_ERR_INVALID;
-    }
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
-    memset(out, 0, sizeof(*out));
-    for (size_t i = 0; i < pl->nrec; ++i) {
-        float ms = 0.f;
-        FDAS_HIP(hipEventElapsedTime(&ms, pl->recs[i].a, pl->recs[i].b));
-        out->launches[pl->recs[i].cat] += 1;
-        out->ms[pl->recs[i].cat] += ms;
-    }
-    return FDAS_OK;
 }

@@ -93,12 +93,13 @@ void h_harmonic_order(int numharm, int* cur, int* max);
 int build_template_bank(cplx* d_bank, int zmax_h, int nz, int L, hipStream_t s);
 
 // ---------------------------------------------------------------- search
-struct SegDesc {       // one (step, harmonic) segment of the overlap-save loop
-    int lobin;         // first spectrum bin fetched
-    int numdata;       // bins normalised (median taken over these)
-    long long out;     // element offset of its L/2 transformed amplitudes
-    int eslot;         // step * H + harmonic: slot of its storage exponent
-    int pad;
+struct SegDesc {          // one (step, harmonic) segment of the overlap-save loop
+    int lobin;            // first spectrum bin fetched
+    int numdata;          // bins normalised (median taken over these)
+    long long out;        // element offset of its L/2 transformed amplitudes
+    int eslot;            // item * H + harmonic: slot of its storage exponent
+    int nbins;            // bins of its trial's spectrum (reads beyond give zero)
+    long long spec_off;   // offset of its trial's spectrum in the spectrum buffer
 };
 
 struct HarmSumArgs {
@@ -117,12 +118,11 @@ struct HarmSumArgs {
     int sub_half;                 // sub-harmonic planes stored as FP16
 };
 
-// Normalise + transform every segment in desc[0..nseg) (all of FFT length L);
-// segment d lands at seg + (d.out - out_base) in the storage format of prec, its
+// Normalise + transform every segment in desc[0..nseg) (all of FFT length L),
+// reading bins spec[d.spec_off + lobin ..] below d.nbins; segment d lands at seg + (d.out - out_base) in the storage format of prec, its
 // exponent (reduced precision) at seg_exp[d.eslot - eslot_base].
 struct SegPrepArgs {
     const cplx* spec;
-    long nbins;
     const SegDesc* desc;
     int nseg;
     void* seg;
@@ -157,12 +157,6 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s);
 // Reduced-precision copy of a bank of Z rows of length L.
 int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
                      hipStream_t s);
-
-// Fundamental version with the harmonic sum, stage thresholds and plane write.
-int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
-                     int step0, int nsteps, int off, const int* d_fullrlo, const int* d_rlo_h,
-                     const float* d_sub, const HarmSumArgs& hs, float* d_plane,
-                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s);
 
 // Harmonic sum in place over d_plane[step][Z0][U] (fundamental powers on entry),
 // stage thresholds and candidate emission.

@@ -50,7 +50,7 @@ __device__ __forceinline__ void hs_emit(fdas_candidate* cands, int* ncand, int c
         c.r_index = col;
         c.stage = stage;
         c.power = p;
-        c.reserved = 0.f;
+        c.trial = 0;
         cands[idx] = c;
     }
 }

@@ -1,0 +1,889 @@
+// plan.hip -- the plan API of include/fdas.h: a batch of DM trials (one or many,
+// of any even lengths) searched with the same zmax / numharm / fftlen.
+//
+// Every (trial, step) pair of the batch is an "item"; the overlap-save loop of
+// AccelSearch.cu:203-247 runs over items in chunks, each chunk being one launch
+// per FFT-length class (segment preparation), one per sub-harmonic and one for
+// the fundamental (fused correlation + inverse FFT + power), and one harmonic-sum
+// launch.  Template banks are shared by every plan of the same configuration
+// (a process-wide registry), so a server keeps one resident copy per zmax.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "fdas_internal.hpp"
+
+using namespace fdas;
+
+namespace fdas {
+
+// ------------------------------------------------------------------ template banks
+struct BankSet {
+    int H = 0;
+    int L[MAX_HARM]{}, Z[MAX_HARM]{};
+    void* bankq[MAX_HARM]{};   // stored form used by the kernels
+    int* bank_exp[MAX_HARM]{};  // per-row exponents (reduced precision)
+    size_t bytes = 0;
+    ~BankSet()
+    {
+        for (int h = 0; h < MAX_HARM; ++h) {
+            (void)hipFree(bankq[h]);
+            (void)hipFree(bank_exp[h]);
+        }
+    }
+};
+
+using BankKey = std::tuple<int, int, int, int, int, int>;   // device, zmax, numharm, uselen, prec
+static std::mutex g_bank_mu;
+static std::map<BankKey, std::weak_ptr<BankSet>> g_banks;
+
+static int build_banks(int zmax, int H, const int* cur, const int* mx, int uselen, int prec,
+                       hipStream_t s, std::shared_ptr<BankSet>* out)
+{
+    int dev = 0;
+    FDAS_HIP(hipGetDevice(&dev));
+    const BankKey key{dev, zmax, H, uselen, prec, 0};
+    std::lock_guard<std::mutex> lk(g_bank_mu);
+    auto it = g_banks.find(key);
+    if (it != g_banks.end()) {
+        if (auto sp = it->second.lock()) {
+            *out = sp;
+            return FDAS_OK;
+        }
+    }
+    auto bs = std::make_shared<BankSet>();
+    bs->H = H;
+    for (int h = 0; h < H; ++h) {
+        const double f = (double)cur[h] / (double)mx[h];
+        const int zmax_h = h_calc_required_z(f, zmax);
+        const int Z = (zmax_h / DZ) * 2 + 1, L = h_fft_length(uselen, mx[h], cur[h], zmax);
+        bs->Z[h] = Z;
+        bs->L[h] = L;
+        cplx* bank = nullptr;
+        FDAS_HIP(hipMalloc(&bank, (size_t)Z * L * sizeof(cplx)));
+        int rc = build_template_bank(bank, zmax_h, Z, L, s);
+        if (rc == FDAS_OK && prec != FDAS_PREC_FP32) {
+            void* q = nullptr;
+            int* e = nullptr;
+            if (hipMalloc(&q, (size_t)Z * L * 4) != hipSuccess ||
+                hipMalloc(&e, (size_t)Z * sizeof(int)) != hipSuccess) {
+                set_error("template bank allocation failed");
+                rc = FDAS_ERR_NOMEM;
+            } else {
+                rc = launch_bank_pack(bank, Z, L, prec, q, e, s);
+            }
+            if (hipStreamSynchronize(s) != hipSuccess && rc == FDAS_OK) rc = FDAS_ERR_HIP;
+            (void)hipFree(bank);
+            bs->bankq[h] = q;
+            bs->bank_exp[h] = e;
+            bs->bytes += (size_t)Z * L * 4 + (size_t)Z * sizeof(int);
+        } else {
+            bs->bankq[h] = bank;
+            bs->bytes += (size_t)Z * L * sizeof(cplx);
+        }
+        FDAS_TRY(rc);
+    }
+    FDAS_HIP(hipStreamSynchronize(s));
+    g_banks[key] = bs;
+    *out = bs;
+    return FDAS_OK;
+}
+
+// ------------------------------------------------------------------ small kernels
+__global__ void zero_int_kernel(int* p) { *p = 0; }
+
+}  // namespace fdas
+
+// ------------------------------------------------------------------ the plan
+struct TrialInfo {
+    long n = 0, nbins = 0;
+    int rlo = 0, nsteps = 0, item0 = 0;
+    long long spec_off = 0;       // offset of its spectrum in d_spec (complex)
+    long long series_off = 0;     // offset of its input in d_series (floats)
+    int form = 0;                 // index of its transform + whitening schedule
+};
+
+// R2C plan and whitening schedule of one series length (shared by its trials).
+struct LengthForm {
+    long n = 0;
+    R2CPlan r2c;
+    WhitenSchedule ws;
+};
+
+struct fdas_plan {
+    fdas_params p{};
+    int uselen = 0, H = 0, Z0 = 0, chunk = 0, nitems = 0;
+    std::vector<TrialInfo> trials;
+    std::vector<LengthForm> forms;
+    int cur[MAX_HARM]{}, mx[MAX_HARM]{}, L[MAX_HARM]{}, Z[MAX_HARM]{}, zmax_h[MAX_HARM]{},
+        off[MAX_HARM]{}, W[MAX_HARM]{}, zlo[MAX_HARM]{};
+    std::shared_ptr<BankSet> banks;
+    int prec = FDAS_PREC_FP32;
+    size_t cbytes = sizeof(cplx);
+    long long seg_off[MAX_HARM]{};
+    long long seg_stride = 0;
+    long long sub_total = 0;
+    long long sub_cells[MAX_HARM]{};
+    long long cells = 0;
+    std::vector<int> classes;
+    std::vector<std::vector<int>> class_harm;
+    std::vector<SegDesc*> d_desc;   // per class: [item][harmonics in class]
+    int* d_fullrlo = nullptr;       // [item]
+    int* d_rlo_h = nullptr;         // [item][MAX_HARM]
+    float* d_series = nullptr;      // plan-owned inputs (batches), sum of n floats
+    cplx* d_spec = nullptr;         // sum of (n/2+1) complex
+    cplx* d_r2c_work = nullptr;
+    void* d_seg = nullptr;
+    int* d_seg_exp = nullptr;
+    float* d_sub = nullptr;
+    float* d_plane = nullptr;
+    fdas_candidate* d_cands = nullptr;
+    int* d_ncand = nullptr;
+    int cand_cap = 0;
+    HarmSumArgs hs{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t bytes = 0;
+    // hipGraph of a whole batch run
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    // event timing
+    bool timing = false;
+    struct Rec {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    size_t nrec = 0;
+};
+
+namespace {
+
+// Brackets one launch with events when timing is on.
+struct TimedLaunch {
+    fdas_plan* pl;
+    size_t idx = (size_t)-1;
+    TimedLaunch(fdas_plan* p, int cat) : pl(p)
+    {
+        if (!pl->timing) return;
+        if (pl->nrec == pl->recs.size()) {
+            fdas_plan::Rec r{cat, nullptr, nullptr};
+            if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+            pl->recs.push_back(r);
+        }
+        idx = pl->nrec++;
+        pl->recs[idx].cat = cat;
+        (void)hipEventRecord(pl->recs[idx].a, pl->stream);
+    }
+    ~TimedLaunch()
+    {
+        if (idx != (size_t)-1) (void)hipEventRecord(pl->recs[idx].b, pl->stream);
+    }
+};
+
+int plan_alloc(fdas_plan* pl, void** ptr, size_t bytes)
+{
+    FDAS_HIP(hipMalloc(ptr, bytes > 0 ? bytes : 1));
+    pl->bytes += bytes;
+    return FDAS_OK;
+}
+
+int validate(const fdas_params& p)
+{
+    if (p.zmax < 0 || p.zmax > 1200 || (p.zmax & 1)) {
+        set_error("zmax %d must be even and in [0, 1200]", p.zmax);
+        return FDAS_ERR_INVALID;
+    }
+    if (p.numharm < 1 || p.numharm > 16 || (p.numharm & (p.numharm - 1))) {
+        set_error("numharm %d must be 1, 2, 4, 8 or 16", p.numharm);
+        return FDAS_ERR_INVALID;
+    }
+    if (p.fftlen < 1024 || p.fftlen > 16384 || (p.fftlen & (p.fftlen - 1))) {
+        set_error("fftlen %d must be a power of two in [1024, 16384]", p.fftlen);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_BF16) {
+        set_error("precision mode %d unknown", p.precision);
+        return FDAS_ERR_INVALID;
+    }
+    return FDAS_OK;
+}
+
+// Host geometry of every item (source/bpd_search.cu:72-127, :139-143) and the
+// checks that the kernels' index arithmetic stays inside the planes.
+int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
+{
+    const fdas_params& p = pl->p;
+    FDAS_TRY(validate(p));
+    pl->prec = p.precision;
+    pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : 4;
+    pl->uselen = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
+    if (pl->uselen <= 0 || (pl->uselen & 1) || pl->uselen >= 65536 ||
+        h_fft_length(pl->uselen, 1, 1, p.zmax) != p.fftlen) {
+        set_error("uselen %d does not give fundamental FFT length %d at zmax %d", pl->uselen,
+                  p.fftlen, p.zmax);
+        return FDAS_ERR_INVALID;
+    }
+    const int U = pl->uselen;
+    const double half = (double)(U / 2);
+    pl->H = p.numharm;
+    h_harmonic_order(pl->H, pl->cur, pl->mx);
+    long long so = 0;
+    for (int h = 0; h < pl->H; ++h) {
+        const double f = (double)pl->cur[h] / (double)pl->mx[h];
+        pl->zmax_h[h] = h_calc_required_z(f, p.zmax);
+        pl->Z[h] = (pl->zmax_h[h] / DZ) * 2 + 1;
+        pl->L[h] = h_fft_length(U, pl->mx[h], pl->cur[h], p.zmax);
+        pl->off[h] = NUMBETWEEN * h_z_response_halfwidth((double)-pl->zmax_h[h]);
+        pl->zlo[h] = h_calc_required_z(f, -p.zmax);
+        pl->seg_off[h] = so;
+        so += pl->L[h] / 2;
+        if (pl->L[h] < 256 || pl->L[h] > 16384) {
+            set_error("harmonic %d/%d needs FFT length %d (fused kernels: 256..16384)",
+                      pl->cur[h], pl->mx[h], pl->L[h]);
+            return FDAS_ERR_UNSUPPORTED;
+        }
+    }
+    pl->seg_stride = so;
+    pl->Z0 = pl->Z[0];
+    for (int h = 1; h < pl->H; ++h) {
+        // rows the harmonic sum reads (monotone in the fundamental row)
+        const int c = pl->cur[h], lm = h_fast_log2(pl->mx[h]);
+        for (int j : {0, pl->Z0 - 1}) {
+            const int zz = pl->zlo[0] + DZ * j;
+            const int x = zz * c;
+            const int subz = x >= 0 ? (x >> lm) : -((-x) >> lm);
+            const int zind = (subz - pl->zlo[h]) >> 1;
+            if (zind < 0 || zind >= pl->Z[h]) {
+                set_error("harmonic-sum row %d outside [0,%d) for harmonic %d", zind, pl->Z[h], h);
+                return FDAS_ERR_INVALID;
+            }
+        }
+    }
+
+    pl->trials.resize(ntrials);
+    long long spec = 0, series = 0;
+    int items = 0;
+    for (int t = 0; t < ntrials; ++t) {
+        const long n = lengths[t];
+        if (n < 256 || (n & 1) || n > (1L << 27)) {
+            set_error("trial %d: series length %ld: even and in [256, 2^27] required", t, n);
+            return FDAS_ERR_UNSUPPORTED;
+        }
+        TrialInfo& tr = pl->trials[t];
+        tr.n = n;
+        tr.nbins = n / 2;
+        // AccelSearch.cu:101: rlo = floor(1.0 * (N * dt)) with a float product
+        const float nd = (float)n * p.dt;
+        tr.rlo = (int)std::floor(1.0 * nd);
+        int ns = 0;
+        for (double st = tr.rlo; st + half < (double)tr.nbins; st += half) ++ns;
+        if (ns <= 0) {
+            set_error("trial %d: no search steps (rlo %d + uselen/2 >= %ld bins)", t, tr.rlo,
+                      tr.nbins);
+            return FDAS_ERR_INVALID;
+        }
+        tr.nsteps = ns;
+        tr.item0 = items;
+        tr.spec_off = spec;
+        tr.series_off = series;
+        items += ns;
+        spec += tr.nbins + 1;
+        series += n;
+    }
+    pl->nitems = items;
+    pl->cells = (long long)items * U * pl->Z0;
+    return FDAS_OK;
+}
+
+int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_h,
+                std::vector<std::vector<SegDesc>>& desc)
+{
+    const int U = pl->uselen, H = pl->H, S = pl->nitems;
+    const double half = (double)(U / 2);
+    fullrlo.assign(S, 0);
+    rlo_h.assign((size_t)S * MAX_HARM, 0);
+    for (int h = 0; h < H; ++h) {
+        pl->W[h] = (h == 0) ? U : 0;
+        pl->sub_cells[h] = 0;
+    }
+    for (int h = 0; h < H; ++h) {
+        auto it = std::find(pl->classes.begin(), pl->classes.end(), pl->L[h]);
+        if (it == pl->classes.end()) {
+            pl->classes.push_back(pl->L[h]);
+            pl->class_harm.push_back({h});
+        } else {
+            pl->class_harm[it - pl->classes.begin()].push_back(h);
+        }
+    }
+    desc.assign(pl->classes.size(), {});
+    for (size_t c = 0; c < pl->classes.size(); ++c) desc[c].resize((size_t)S * pl->class_harm[c].size());
+    for (const TrialInfo& tr : pl->trials) {
+        for (int k = 0; k < tr.nsteps; ++k) {
+            const int item = tr.item0 + k;
+            const double startr = tr.rlo + (double)k * half;
+            const double lastr = startr + half - DR;
+            fullrlo[item] = (int)startr;
+            for (int h = 0; h < H; ++h) {
+                const double f = (double)pl->cur[h] / (double)pl->mx[h];
+                const double drlo = h_calc_required_r(f, startr);
+                const double drhi = h_calc_required_r(f, lastr);
+                const int rl = (int)std::floor(drlo);
+                const int bo = pl->off[h] / NUMBETWEEN;
+                int nr = U;
+                if (h > 0) {
+                    nr = (int)((std::ceil(drhi) - std::floor(drlo)) * RDR + DBLCORRECT) + 1;
+                    if (nr % RDR) nr = (nr / RDR + 1) * RDR;
+                    // columns the harmonic sum reads at the first and last fundamental column
+                    const double s0 = (int)(RDR * (fullrlo[item] + 0 * DR) * f + 0.5) * DR;
+                    const double s1 = (int)(RDR * (fullrlo[item] + (U - 1) * DR) * f + 0.5) * DR;
+                    const int r0 = (int)((s0 - rl) * RDR), r1 = (int)((s1 - rl) * RDR);
+                    if (r0 < 0 || r1 >= nr) {
+                        set_error("harmonic-sum index %d..%d outside [0,%d) at step %d", r0, r1, nr, k);
+                        return FDAS_ERR_INVALID;
+                    }
+                }
+                pl->W[h] = std::max(pl->W[h], nr);
+                pl->sub_cells[h] += (long long)pl->Z[h] * nr;
+                rlo_h[(size_t)item * MAX_HARM + h] = rl;
+                const int lobin = rl - bo;
+                const int hibin = (int)std::ceil(drhi) + bo;
+                const int numdata = hibin - lobin + 1;
+                if (h_next_pow2(numdata) != pl->L[h] / 2) {
+                    set_error("step %d harmonic %d/%d: %d bins to normalise do not round to fftlen/2 = %d",
+                              k, pl->cur[h], pl->mx[h], numdata, pl->L[h] / 2);
+                    return FDAS_ERR_UNSUPPORTED;
+                }
+                for (size_t c = 0; c < pl->classes.size(); ++c) {
+                    const auto& hl = pl->class_harm[c];
+                    auto q = std::find(hl.begin(), hl.end(), h);
+                    if (q == hl.end()) continue;
+                    SegDesc& e = desc[c][(size_t)item * hl.size() + (q - hl.begin())];
+                    e.lobin = lobin;
+                    e.numdata = numdata;
+                    e.out = (long long)item * pl->seg_stride + pl->seg_off[h];
+                    e.eslot = item * H + h;
+                    e.nbins = (int)tr.nbins;
+                    e.spec_off = tr.spec_off;
+                }
+            }
+        }
+    }
+    long long sub = 0;
+    for (int h = 0; h < H; ++h) {
+        if (pl->off[h] + pl->W[h] > pl->L[h]) {
+            set_error("harmonic %d: %d columns from offset %d exceed FFT length %d", h, pl->W[h],
+                      pl->off[h], pl->L[h]);
+            return FDAS_ERR_INVALID;
+        }
+        if (h > 0) sub += (long long)pl->Z[h] * pl->W[h];
+    }
+    pl->sub_total = sub;
+    return FDAS_OK;
+}
+
+int plan_build(fdas_plan* pl, const long* lengths, int ntrials, bool own_series)
+{
+    FDAS_TRY(plan_geometry(pl, lengths, ntrials));
+    std::vector<int> fullrlo, rlo_h;
+    std::vector<std::vector<SegDesc>> desc;
+    FDAS_TRY(plan_tables(pl, fullrlo, rlo_h, desc));
+    const fdas_params& p = pl->p;
+    const int U = pl->uselen, H = pl->H;
+
+    // chunking: per-item buffer bytes against a device-memory budget
+    const size_t per_item = (size_t)pl->seg_stride * pl->cbytes + (size_t)pl->sub_total * sizeof(float) +
+                            (size_t)pl->Z0 * U * sizeof(float) + (size_t)H * sizeof(int);
+    int chunk = p.chunk_steps;
+    if (chunk <= 0) chunk = (int)std::max<size_t>(1, ((size_t)16 << 30) / per_item);
+    pl->chunk = std::min(chunk, pl->nitems);
+
+    // device state (everything above is host-side validation and geometry)
+    FDAS_HIP(hipStreamCreate(&pl->stream));
+    pl->own_stream = true;
+    FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, U, pl->prec, pl->stream, &pl->banks));
+    for (size_t c = 0; c < desc.size(); ++c) {
+        SegDesc* dd = nullptr;
+        FDAS_TRY(plan_alloc(pl, (void**)&dd, desc[c].size() * sizeof(SegDesc)));
+        pl->d_desc.push_back(dd);
+        FDAS_HIP(hipMemcpy(dd, desc[c].data(), desc[c].size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+    }
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_fullrlo, fullrlo.size() * sizeof(int)));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_rlo_h, rlo_h.size() * sizeof(int)));
+    FDAS_HIP(hipMemcpy(pl->d_fullrlo, fullrlo.data(), fullrlo.size() * sizeof(int), hipMemcpyHostToDevice));
+    FDAS_HIP(hipMemcpy(pl->d_rlo_h, rlo_h.data(), rlo_h.size() * sizeof(int), hipMemcpyHostToDevice));
+    long long spec = 0, series = 0;
+    size_t work = 0;
+    for (TrialInfo& tr : pl->trials) {
+        spec += tr.nbins + 1;
+        series += tr.n;
+        size_t f = 0;
+        while (f < pl->forms.size() && pl->forms[f].n != tr.n) ++f;
+        if (f == pl->forms.size()) {
+            pl->forms.emplace_back();
+            pl->forms.back().n = tr.n;
+            FDAS_TRY(r2c_plan_create(tr.n, &pl->forms.back().r2c, pl->stream));
+            FDAS_TRY(whiten_schedule_create((unsigned)tr.nbins, &pl->forms.back().ws));
+            work = std::max(work, pl->forms.back().r2c.work_elems);
+        }
+        tr.form = (int)f;
+    }
+    if (own_series) FDAS_TRY(plan_alloc(pl, (void**)&pl->d_series, (size_t)series * sizeof(float)));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_spec, (size_t)spec * sizeof(cplx)));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_r2c_work, work * sizeof(cplx)));
+    FDAS_TRY(plan_alloc(pl, &pl->d_seg, (size_t)pl->chunk * pl->seg_stride * pl->cbytes));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg_exp, (size_t)pl->chunk * H * sizeof(int)));
+    if (pl->sub_total > 0)
+        FDAS_TRY(plan_alloc(pl, (void**)&pl->d_sub, (size_t)pl->chunk * pl->sub_total * sizeof(float)));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_plane, (size_t)pl->chunk * pl->Z0 * U * sizeof(float)));
+    pl->cand_cap = p.max_candidates > 0 ? p.max_candidates : (1 << 20);
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_cands, (size_t)pl->cand_cap * sizeof(fdas_candidate)));
+    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_ncand, sizeof(int)));
+    FDAS_HIP(hipMemsetAsync(pl->d_ncand, 0, sizeof(int), pl->stream));
+
+    // harmonic-sum arguments
+    HarmSumArgs& hs = pl->hs;
+    hs.H = H;
+    hs.U = U;
+    hs.zlo0 = pl->zlo[0];
+    long long acc = 0;
+    for (int h = 0; h < H; ++h) {
+        hs.c[h] = pl->cur[h];
+        hs.lm[h] = h_fast_log2(pl->mx[h]);
+        hs.zlo[h] = pl->zlo[h];
+        hs.Z[h] = pl->Z[h];
+        hs.W[h] = pl->W[h];
+        if (h > 0) {
+            hs.sub_off[h] = acc;
+            hs.sub_stride[h] = (long long)pl->Z[h] * pl->W[h];
+            acc += (long long)pl->chunk * hs.sub_stride[h];
+        }
+    }
+    for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
+    hs.cand_cap = pl->cand_cap;
+    hs.sub_half = 0;
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
+    return FDAS_OK;
+}
+
+int prepare_trial(fdas_plan* pl, int t, const float* d_series)
+{
+    const TrialInfo& tr = pl->trials[t];
+    const LengthForm& lf = pl->forms[tr.form];
+    {
+        TimedLaunch tl(pl, FDAS_K_R2C);
+        FDAS_TRY(fft_r2c_plan(lf.r2c, d_series, pl->d_spec + tr.spec_off, pl->d_r2c_work, pl->stream));
+    }
+    TimedLaunch tl(pl, FDAS_K_WHITEN);
+    return whiten(pl->d_spec + tr.spec_off, lf.ws, pl->stream);
+}
+
+int reset_candidates(fdas_plan* pl)
+{
+    // a kernel, not a memset: this runs inside captured graphs too
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, pl->stream, pl->d_ncand);
+    return check_launch("zero_int_kernel");
+}
+
+int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
+{
+    const int U = pl->uselen;
+    const BankSet& bs = *pl->banks;
+    for (int c0 = item0; c0 < item0 + nitems; c0 += pl->chunk) {
+        const int cs = std::min(pl->chunk, item0 + nitems - c0);
+        for (size_t c = 0; c < pl->classes.size(); ++c) {
+            const int nh = (int)pl->class_harm[c].size();
+            SegPrepArgs a{};
+            a.spec = pl->d_spec;
+            a.desc = pl->d_desc[c] + (size_t)c0 * nh;
+            a.nseg = cs * nh;
+            a.seg = pl->d_seg;
+            // descriptors hold offsets as if every item had a buffer slot
+            a.out_base = (long long)c0 * pl->seg_stride;
+            a.seg_exp = pl->d_seg_exp;
+            a.eslot_base = c0 * pl->H;
+            a.prec = pl->prec;
+            TimedLaunch tl(pl, FDAS_K_SEGPREP);
+            FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
+        }
+        auto corr_args = [&](int h, void* out, int W) {
+            CorrArgs a{};
+            a.bank = bs.bankq[h];
+            a.bank_exp = bs.bank_exp[h];
+            a.Z = pl->Z[h];
+            a.seg = pl->d_seg;
+            a.seg_stride = pl->seg_stride;
+            a.seg_off = pl->seg_off[h];
+            a.seg_exp = pl->d_seg_exp;
+            a.H = pl->H;
+            a.h = h;
+            a.nsteps = cs;
+            a.off = pl->off[h];
+            a.W = W;
+            a.out = out;
+            a.out_half = 0;
+            a.prec = pl->prec;
+            return a;
+        };
+        for (int h = 1; h < pl->H; ++h) {
+            TimedLaunch tl(pl, FDAS_K_CORR_SUB);
+            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, pl->d_sub + pl->hs.sub_off[h], pl->W[h]),
+                                 pl->stream));
+        }
+        float* plane = d_planes ? d_planes + (size_t)(c0 - item0) * pl->Z0 * U : pl->d_plane;
+        {
+            TimedLaunch tl(pl, FDAS_K_CORR_FUND);
+            FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream));
+        }
+        TimedLaunch tl(pl, FDAS_K_HARMSUM);
+        FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo, pl->d_rlo_h,
+                                pl->d_cands, pl->d_ncand, pl->stream));
+    }
+    return FDAS_OK;
+}
+
+// Every trial from the plan's own series buffers, then every item.
+int batch_body(fdas_plan* pl)
+{
+    for (size_t t = 0; t < pl->trials.size(); ++t)
+        FDAS_TRY(prepare_trial(pl, (int)t, pl->d_series + pl->trials[t].series_off));
+    FDAS_TRY(reset_candidates(pl));
+    return run_items(pl, 0, pl->nitems, nullptr);
+}
+
+int create_plan(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
+                fdas_plan** out)
+{
+    if (!params || !out || ntrials < 1 || !lengths) {
+        set_error("fdas plan: null argument or no trials");
+        return FDAS_ERR_INVALID;
+    }
+    *out = nullptr;
+    fdas_plan* pl = new fdas_plan();
+    pl->p = *params;
+    const int rc = plan_build(pl, lengths, ntrials, own_series);
+    if (rc != FDAS_OK) {
+        fdas_plan_destroy(pl);
+        return rc;
+    }
+    *out = pl;
+    return FDAS_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" void fdas_plan_destroy(fdas_plan* pl)
+{
+    if (!pl) return;
+    if (pl->stream) (void)hipStreamSynchronize(pl->stream);
+    if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
+    for (SegDesc* d : pl->d_desc) (void)hipFree(d);
+    (void)hipFree(pl->d_fullrlo);
+    (void)hipFree(pl->d_rlo_h);
+    (void)hipFree(pl->d_series);
+    (void)hipFree(pl->d_spec);
+    (void)hipFree(pl->d_r2c_work);
+    for (LengthForm& lf : pl->forms) {
+        r2c_plan_destroy(&lf.r2c);
+        whiten_schedule_destroy(&lf.ws);
+    }
+    (void)hipFree(pl->d_seg);
+    (void)hipFree(pl->d_seg_exp);
+    (void)hipFree(pl->d_sub);
+    (void)hipFree(pl->d_plane);
+    (void)hipFree(pl->d_cands);
+    (void)hipFree(pl->d_ncand);
+    for (auto& r : pl->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    pl->banks.reset();
+    if (pl->own_stream) (void)hipStreamDestroy(pl->stream);
+    delete pl;
+}
+
+extern "C" int fdas_plan_create(const fdas_params* params, fdas_plan** out)
+{
+    if (!params) {
+        set_error("fdas_plan_create: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    const long n = params->n;
+    return create_plan(params, 1, &n, false, out);
+}
+
+extern "C" int fdas_batch_create(const fdas_params* params, int ntrials, const long* lengths,
+                                 fdas_plan** out)
+{
+    return create_plan(params, ntrials, lengths, true, out);
+}
+
+extern "C" int fdas_plan_get_info(const fdas_plan* pl, fdas_plan_info* info)
+{
+    if (!pl || !info) {
+        set_error("fdas_plan_get_info: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    memset(info, 0, sizeof(*info));
+    info->uselen = pl->uselen;
+    info->rlo = pl->trials[0].rlo;
+    info->nsteps = pl->nitems;
+    info->numharm = pl->H;
+    info->numz = pl->Z0;
+    for (int h = 0; h < pl->H; ++h) {
+        info->fftlen[h] = pl->L[h];
+        info->numzs[h] = pl->Z[h];
+        info->harm_cur[h] = pl->cur[h];
+        info->harm_max[h] = pl->mx[h];
+        info->sub_cells[h] = pl->sub_cells[h];
+    }
+    info->chunk_steps = pl->chunk;
+    info->nbins = pl->trials[0].nbins;
+    info->cells = pl->cells;
+    info->device_bytes = pl->bytes;
+    info->ntrials = (int)pl->trials.size();
+    info->template_bytes = pl->banks ? pl->banks->bytes : 0;
+    return FDAS_OK;
+}
+
+extern "C" int fdas_batch_trial_info(const fdas_plan* pl, int trial, fdas_trial_info* info)
+{
+    if (!pl || !info || trial < 0 || trial >= (int)pl->trials.size()) {
+        set_error("fdas_batch_trial_info: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    const TrialInfo& tr = pl->trials[trial];
+    info->n = tr.n;
+    info->nbins = tr.nbins;
+    info->rlo = tr.rlo;
+    info->nsteps = tr.nsteps;
+    info->first_step = tr.item0;
+    return FDAS_OK;
+}
+
+extern "C" float* fdas_batch_series(fdas_plan* pl, int trial)
+{
+    if (!pl || !pl->d_series || trial < 0 || trial >= (int)pl->trials.size()) {
+        set_error("fdas_batch_series: no plan-owned series for trial %d", trial);
+        return nullptr;
+    }
+    return pl->d_series + pl->trials[trial].series_off;
+}
+
+extern "C" int fdas_plan_set_stream(fdas_plan* pl, void* stream)
+{
+    if (!pl) {
+        set_error("fdas_plan_set_stream: null plan");
+        return FDAS_ERR_INVALID;
+    }
+    if (pl->own_stream) {
+        (void)hipStreamSynchronize(pl->stream);
+        (void)hipStreamDestroy(pl->stream);
+        pl->own_stream = false;
+    }
+    if (pl->graph) {
+        (void)hipGraphExecDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    if (stream) {
+        pl->stream = (hipStream_t)stream;
+    } else {
+        FDAS_HIP(hipStreamCreate(&pl->stream));
+        pl->own_stream = true;
+    }
+    return FDAS_OK;
+}
+
+extern "C" int fdas_prepare_trial(fdas_plan* pl, int trial, const float* d_series)
+{
+    if (!pl || trial < 0 || trial >= (int)pl->trials.size()) {
+        set_error("fdas_prepare_trial: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    if (!d_series) d_series = fdas_batch_series(pl, trial);
+    if (!d_series) return FDAS_ERR_INVALID;
+    return prepare_trial(pl, trial, d_series);
+}
+
+extern "C" int fdas_prepare(fdas_plan* pl, const float* d_series)
+{
+    if (!pl || !d_series) {
+        set_error("fdas_prepare: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    if (pl->trials.size() != 1) {
+        set_error("fdas_prepare: plan has %zu trials (use fdas_prepare_trial)", pl->trials.size());
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_TRY(prepare_trial(pl, 0, d_series));
+    return reset_candidates(pl);
+}
+
+extern "C" int fdas_reset_candidates(fdas_plan* pl)
+{
+    if (!pl) {
+        set_error("fdas_reset_candidates: null plan");
+        return FDAS_ERR_INVALID;
+    }
+    return reset_candidates(pl);
+}
+
+extern "C" int fdas_run_steps(fdas_plan* pl, int step0, int nsteps, float* d_planes)
+{
+    if (!pl || step0 < 0 || nsteps < 0 || step0 + nsteps > pl->nitems) {
+        set_error("fdas_run_steps: steps [%d, %d) outside [0, %d)", step0, step0 + nsteps,
+                  pl ? pl->nitems : 0);
+        return FDAS_ERR_INVALID;
+    }
+    return run_items(pl, step0, nsteps, d_planes);
+}
+
+extern "C" int fdas_search(fdas_plan* pl, const float* d_series)
+{
+    FDAS_TRY(fdas_prepare(pl, d_series));
+    return run_items(pl, 0, pl->nitems, nullptr);
+}
+
+extern "C" int fdas_plan_enable_graph(fdas_plan* pl, int enable)
+{
+    if (!pl) {
+        set_error("fdas_plan_enable_graph: null plan");
+        return FDAS_ERR_INVALID;
+    }
+    if (!pl->d_series) {
+        set_error("graph replay needs a batch plan (plan-owned series)");
+        return FDAS_ERR_INVALID;
+    }
+    pl->use_graph = enable != 0;
+    if (!pl->use_graph && pl->graph) {
+        (void)hipGraphExecDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    return FDAS_OK;
+}
+
+extern "C" int fdas_batch_run(fdas_plan* pl)
+{
+    if (!pl || !pl->d_series) {
+        set_error("fdas_batch_run: not a batch plan");
+        return FDAS_ERR_INVALID;
+    }
+    if (!pl->use_graph || pl->timing) return batch_body(pl);
+    if (!pl->graph) {
+        // one eager run creates the lazily built twiddle tables outside the capture
+        FDAS_TRY(batch_body(pl));
+        FDAS_HIP(hipStreamSynchronize(pl->stream));
+        hipGraph_t g = nullptr;
+        FDAS_HIP(hipStreamBeginCapture(pl->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = batch_body(pl);
+        hipGraph_t cap = nullptr;
+        const hipError_t e = hipStreamEndCapture(pl->stream, &cap);
+        FDAS_TRY(rc);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        g = cap;
+        const hipError_t e2 = hipGraphInstantiate(&pl->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) return hip_fail(e2, "hipGraphInstantiate", __FILE__, __LINE__);
+        return FDAS_OK;
+    }
+    FDAS_HIP(hipGraphLaunch(pl->graph, pl->stream));
+    return FDAS_OK;
+}
+
+extern "C" int fdas_get_spectrum(fdas_plan* pl, fdas_complex* d_out)
+{
+    return fdas_batch_spectrum(pl, 0, d_out);
+}
+
+extern "C" int fdas_batch_spectrum(fdas_plan* pl, int trial, fdas_complex* d_out)
+{
+    if (!pl || !d_out || trial < 0 || trial >= (int)pl->trials.size()) {
+        set_error("fdas_batch_spectrum: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    const TrialInfo& tr = pl->trials[trial];
+    FDAS_HIP(hipMemcpyAsync(d_out, pl->d_spec + tr.spec_off, (size_t)tr.nbins * sizeof(cplx),
+                            hipMemcpyDeviceToDevice, pl->stream));
+    return FDAS_OK;
+}
+
+extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capacity, int* count,
+                                   int* overflowed)
+{
+    if (!pl || !count) {
+        set_error("fdas_get_candidates: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
+    int n = 0;
+    FDAS_HIP(hipMemcpy(&n, pl->d_ncand, sizeof(int), hipMemcpyDeviceToHost));
+    const int kept = std::min(n, pl->cand_cap);
+    if (overflowed) *overflowed = n > pl->cand_cap;
+    std::vector<fdas_candidate> all(kept);
+    if (kept > 0)
+        FDAS_HIP(hipMemcpy(all.data(), pl->d_cands, (size_t)kept * sizeof(fdas_candidate),
+                           hipMemcpyDeviceToHost));
+    // kernels record the item; report (trial, step within the trial)
+    for (auto& c : all) {
+        int t = (int)pl->trials.size() - 1;
+        while (t > 0 && pl->trials[t].item0 > c.step) --t;
+        c.trial = t;
+        c.step -= pl->trials[t].item0;
+    }
+    std::sort(all.begin(), all.end(), [](const fdas_candidate& a, const fdas_candidate& b) {
+        if (a.trial != b.trial) return a.trial < b.trial;
+        if (a.step != b.step) return a.step < b.step;
+        if (a.z_index != b.z_index) return a.z_index < b.z_index;
+        if (a.r_index != b.r_index) return a.r_index < b.r_index;
+        return a.stage < b.stage;
+    });
+    *count = kept;
+    if (out && capacity > 0) memcpy(out, all.data(), sizeof(fdas_candidate) * std::min(kept, capacity));
+    return FDAS_OK;
+}
+
+extern "C" int fdas_synchronize(fdas_plan* pl)
+{
+    if (!pl) {
+        set_error("fdas_synchronize: null plan");
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
+    return FDAS_OK;
+}
+
+extern "C" int fdas_plan_enable_timing(fdas_plan* pl, int enable)
+{
+    if (!pl) {
+        set_error("fdas_plan_enable_timing: null plan");
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
+    pl->timing = enable != 0;
+    pl->nrec = 0;
+    return FDAS_OK;
+}
+
+extern "C" int fdas_plan_get_timing(fdas_plan* pl, fdas_timing* out)
+{
+    if (!pl || !out) {
+        set_error("fdas_plan_get_timing: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
+    memset(out, 0, sizeof(*out));
+    for (size_t i = 0; i < pl->nrec; ++i) {
+        float ms = 0.f;
+        FDAS_HIP(hipEventElapsedTime(&ms, pl->recs[i].a, pl->recs[i].b));
+        out->launches[pl->recs[i].cat] += 1;
+        out->ms[pl->recs[i].cat] += ms;
+    }
+    return FDAS_OK;
+}

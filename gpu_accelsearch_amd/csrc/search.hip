@@ -9,9 +9,8 @@
 //   corr_sub_kernel   per (template, step) of a sub-harmonic: multiply by the
 //                     conjugate template, inverse FFT of length L in LDS, |y|^2/L^2
 //                     of the kept columns -> sub-harmonic power plane.
-//   corr_fund_kernel  the same for the fundamental, then the harmonic sum of
-//                     source/bpd_gpu_kernels.cu:119-170 in the reference order,
-//                     per-stage thresholds, and one write of the summed plane.
+//                     The fundamental uses the same kernel with its own plane;
+//                     harmsum.hip then adds the sub-harmonic planes in place.
 // The complex f-fdot plane is never written to memory.
 //
 // Stage path (reference data layout, source/bpd_search.cu:228-360): convolve,
@@ -126,7 +125,7 @@ __device__ __forceinline__ float block_max_abs(const cplx* v, float* red, int t)
 
 template <int NH, int P>
 __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
-    const cplx* __restrict__ spec, long nbins, const SegDesc* __restrict__ desc,
+    const cplx* __restrict__ spec_all, const SegDesc* __restrict__ desc,
     typename CStore<P>::T* __restrict__ seg, long long out_base, int* __restrict__ seg_exp,
     int eslot_base, const cplx* __restrict__ tw)
 {
@@ -137,6 +136,7 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     __shared__ int sel[2];
     __shared__ float red[T / 64 + 1];
     const SegDesc d = desc[blockIdx.x];
+    const cplx* __restrict__ spec = spec_all + d.spec_off;
     const int t = threadIdx.x;
     cplx v[F::E];
     unsigned pb[F::E];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     for (int i = 0; i < F::E; ++i) {
         const int n = F::in_index(t, i);
         const long b = (long)d.lobin + n;
-        v[i] = (b >= 0 && b < nbins) ? spec[b] : mk(0.f, 0.f);
+        v[i] = (b >= 0 && b < d.nbins) ? spec[b] : mk(0.f, 0.f);
         pb[i] = __float_as_uint(power_nc(v[i]));
     }
     // exact lower median: rank (numdata-1)/2 among the first numdata powers
@@ -275,179 +275,6 @@ __global__ __launch_bounds__(256) void bank_pack_kernel(const cplx* __restrict__
         out[(size_t)z * L + i] = CStore<P>::put(mk(ldexpf(row[i].x, -e), ldexpf(row[i].y, -e)));
 }
 
-__device__ __forceinline__ void emit_candidate(fdas_candidate* cands, int* ncand, int cap,
-                                               int step, int z, int col, int stage, float p)
-{
-    const int idx = atomicAdd(ncand, 1);
-    if (idx < cap) {
-        fdas_candidate c;
-        c.step = step;
-        c.z_index = z;
-        c.r_index = col;
-        c.stage = stage;
-        c.power = p;
-        c.reserved = 0.f;
-        cands[idx] = c;
-    }
-}
-
-// Sub-harmonic column read by fundamental column col (source/bpd_gpu_kernels.cu:139-141):
-// subr*2 = floor((2 rr) f + 1/2) with f = c/2^lm and q = 2 rr = 2 fullrlo + col.
-// q*c < 2^31 for series up to 2^27 samples and c < 16.
-__device__ __forceinline__ int sub_rind(unsigned q, int c, int lm, int rlo2)
-{
-    return (int)((q * (unsigned)c + (1u << (lm - 1))) >> lm) - rlo2;
-}
-
-// Sub-harmonic row read by fundamental row zz = zlo + 2j: (int)(zz f) truncates
-// toward zero, then (subz - zlo_h) / 2 (source/bpd_gpu_kernels.cu:147-153).
-__device__ __forceinline__ int sub_zind(int zz, int c, int lm, int zlo_h)
-{
-    const int x = zz * c;
-    const int subz = x >= 0 ? (x >> lm) : -((-x) >> lm);
-    return (subz - zlo_h) >> 1;
-}
-
-// Fundamental: y = IFFT(A * conj K), power, then the harmonic sum in the reference
-// order.  The sub-harmonic row segments this workgroup needs (row zind_h, columns
-// rind(0)..rind(U-1), about U*f_h floats each) are staged in the FFT's LDS buffer,
-// as many harmonics at a time as fit, with batches of independent loads, and
-// gathered from there.  Stage thresholds mark outputs in one bit mask; the rare
-// candidates are emitted afterwards with their partial sums recomputed.
-template <int L>
-__global__ __launch_bounds__(FftCfgS<L>::T) void corr_fund_kernel(
-    const cplx* __restrict__ bank, int Z, const cplx* __restrict__ seg, long long seg_stride,
-    int step0, int off, const int* __restrict__ fullrlo, const int* __restrict__ rlo_h,
-    const float* __restrict__ sub, HarmSumArgs hs, float* __restrict__ plane,
-    fdas_candidate* __restrict__ cands, int* __restrict__ ncand, const cplx* __restrict__ tw,
-    float scale)
-{
-    constexpr int T = FftCfgS<L>::T;
-    using F = BlockFFT<L, T, 1>;
-    constexpr int E = F::E;
-    constexpr int LDS_FLOATS = 2 * F::LDS_ELEMS;
-    constexpr int BATCH = 16;
-    static_assert(E <= 32, "the hit mask holds 32 outputs per thread");
-    __shared__ cplx lds[F::LDS_ELEMS];
-    __shared__ long long meta_src[MAX_HARM];
-    __shared__ int meta_base[MAX_HARM + 1];
-    float* ldsf = reinterpret_cast<float*>(lds);
-    const int z = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
-    const int gstep = step0 + s;
-    const int zz = hs.zlo0 + DZ * z;
-    const int U = hs.U;
-    cplx v[E];
-    {
-        cplx kr[E];
-        load_template<L>(bank + (size_t)z * L, kr, t);
-        corr_ifft<L>(kr, seg + s * seg_stride, tw, lds, v, t);
-    }
-    const unsigned q0 = 2u * (unsigned)fullrlo[gstep];
-    const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
-    float acc[E], p0[E];
-    unsigned hit = 0;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int col = F::out_index(t, i) - off;
-        p0[i] = (v[i].x * v[i].x + v[i].y * v[i].y) * scale;
-        acc[i] = p0[i];
-        if (col >= 0 && col < U && p0[i] > hs.powcut[0]) hit |= 1u << i;
-    }
-    int stage = 0;
-    for (int h = 1; h < hs.H;) {
-        // harmonics [h, h2) whose row segments fit in LDS together
-        int h2 = h, used = 0;
-        while (h2 < hs.H) {
-            const int r2 = 2 * rl[h2];
-            const int len = sub_rind(q0 + U - 1, hs.c[h2], hs.lm[h2], r2) -
-                            sub_rind(q0, hs.c[h2], hs.lm[h2], r2) + 1;
-            if (used + len > LDS_FLOATS) break;
-            used += len;
-            ++h2;
-        }
-        __syncthreads();   // earlier readers of the LDS buffer are done
-        if (t == 0) {
-            int base = 0;
-            for (int g = h; g < h2; ++g) {
-                const int r2 = 2 * rl[g];
-                const int lo = sub_rind(q0, hs.c[g], hs.lm[g], r2);
-                const int zind = sub_zind(zz, hs.c[g], hs.lm[g], hs.zlo[g]);
-                meta_src[g - h] = hs.sub_off[g] + s * hs.sub_stride[g] +
-                                  (long long)zind * hs.W[g] + lo - base;
-                meta_base[g - h] = base;
-                base += sub_rind(q0 + U - 1, hs.c[g], hs.lm[g], r2) - lo + 1;
-            }
-            meta_base[h2 - h] = base;
-        }
-        __syncthreads();
-        const int ng = h2 - h;
-        for (int k0 = t; k0 < used; k0 += BATCH * T) {
-            float r[BATCH];
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const int k = k0 + u * T;
-                int g = 0;
-                while (g + 1 < ng && k >= meta_base[g + 1]) ++g;
-                r[u] = k < used ? sub[meta_src[g] + k] : 0.f;
-            }
-#pragma unroll
-            for (int u = 0; u < BATCH; ++u) {
-                const int k = k0 + u * T;
-                if (k < used) ldsf[k] = r[u];
-            }
-        }
-        __syncthreads();
-        for (int g = h; g < h2; ++g) {
-            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
-            const int shift = meta_base[g - h] - sub_rind(q0, c, lm, r2);
-#pragma unroll
-            for (int i = 0; i < E; ++i) {
-                int col = F::out_index(t, i) - off;
-                col = col < 0 ? 0 : (col >= U ? U - 1 : col);
-                acc[i] += ldsf[shift + sub_rind(q0 + col, c, lm, r2)];
-            }
-            if (((g + 1) & g) == 0) {
-                ++stage;
-                const float pc = hs.powcut[stage];
-#pragma unroll
-                for (int i = 0; i < E; ++i) {
-                    const int col = F::out_index(t, i) - off;
-                    if (col >= 0 && col < U && acc[i] > pc) hit |= 1u << i;
-                }
-            }
-        }
-        h = h2;
-    }
-    float* o = plane + ((size_t)s * Z + z) * U;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-        const int col = F::out_index(t, i) - off;
-        if (col >= 0 && col < U) o[col] = acc[i];
-    }
-    // rare path: the marked outputs' stage sums, re-accumulated in the same order
-    if (hit) {
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            if (!((hit >> i) & 1u)) continue;
-            const int col = F::out_index(t, i) - off;
-            float a2 = p0[i];
-            int st = 0;
-            if (a2 > hs.powcut[0]) emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, 0, a2);
-            for (int h = 1; h < hs.H; ++h) {
-                const int c = hs.c[h], lm = hs.lm[h];
-                const int rind = sub_rind(q0 + col, c, lm, 2 * rl[h]);
-                const int zind = sub_zind(zz, c, lm, hs.zlo[h]);
-                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
-                if (((h + 1) & h) == 0) {
-                    ++st;
-                    if (a2 > hs.powcut[st])
-                        emit_candidate(cands, ncand, hs.cand_cap, gstep, z, col, st, a2);
-                }
-            }
-        }
-    }
-}
-
 // ------------------------------------------------------------------ launchers
 template <int NH, int P>
 static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
@@ -460,7 +287,7 @@ static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
         return FDAS_ERR_UNSUPPORTED;
     }
     hipLaunchKernelGGL((seg_prep_kernel<NH, P>), dim3(a.nseg), dim3(FftCfgS<NH>::T), 0, s, a.spec,
-                       a.nbins, a.desc, static_cast<typename CStore<P>::T*>(a.seg), a.out_base,
+                       a.desc, static_cast<typename CStore<P>::T*>(a.seg), a.out_base,
                        a.seg_exp, a.eslot_base, tw);
     return check_launch("seg_prep_kernel");
 }
@@ -566,42 +393,6 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
         return FDAS_ERR_INVALID;
     }
     return check_launch("bank_pack_kernel");
-}
-
-template <int L>
-static int corr_fund_t(const cplx* bank, int Z, const cplx* seg, long long seg_stride, int step0,
-                       int nsteps, int off, const int* fullrlo, const int* rlo_h, const float* sub,
-                       const HarmSumArgs& hs, float* plane, fdas_candidate* cands, int* ncand,
-                       hipStream_t s)
-{
-    int err = FDAS_OK;
-    const cplx* tw = fft_twiddles(L, &err);
-    if (!tw) return err;
-    const float scale = (float)(1.0 / ((double)L * (double)L));
-    hipLaunchKernelGGL(corr_fund_kernel<L>, dim3(Z, nsteps), dim3(FftCfgS<L>::T), 0, s, bank, Z,
-                       seg, seg_stride, step0, off, fullrlo, rlo_h, sub, hs, plane, cands, ncand,
-                       tw, scale);
-    return check_launch("corr_fund_kernel");
-}
-
-int launch_corr_fund(int L, const cplx* d_bank, int Z, const cplx* d_seg, long long seg_stride,
-                     int step0, int nsteps, int off, const int* d_fullrlo, const int* d_rlo_h,
-                     const float* d_sub, const HarmSumArgs& hs, float* d_plane,
-                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
-{
-    if (nsteps <= 0) return FDAS_OK;
-    switch (L) {
-        case 256: return corr_fund_t<256>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 512: return corr_fund_t<512>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 1024: return corr_fund_t<1024>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 2048: return corr_fund_t<2048>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 4096: return corr_fund_t<4096>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 8192: return corr_fund_t<8192>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        case 16384: return corr_fund_t<16384>(d_bank, Z, d_seg, seg_stride, step0, nsteps, off, d_fullrlo, d_rlo_h, d_sub, hs, d_plane, d_cands, d_ncand, s);
-        default:
-            set_error("fused correlation length %d unsupported", L);
-            return FDAS_ERR_UNSUPPORTED;
-    }
 }
 
 // ------------------------------------------------------------------ stage path

@@ -177,16 +177,28 @@ typedef struct fdas_plan_info {
     long nbins;                        /* n/2: bins whitened and searched */
     long long cells;                   /* nsteps * uselen * numz          */
     long long sub_cells[FDAS_MAX_HARM]; /* sum over steps of numzs*numrs   */
-    size_t device_bytes;
+    size_t device_bytes;               /* plan-owned buffers              */
+    int ntrials;                       /* DM trials in the plan (1 = single) */
+    size_t template_bytes;             /* shared template banks           */
 } fdas_plan_info;
+/* In a batch plan rlo and nbins are those of trial 0, nsteps and cells are
+ * summed over all trials. */
+
+typedef struct fdas_trial_info {
+    long n;          /* series length                                      */
+    long nbins;      /* n/2                                                */
+    int rlo;         /* floor(n*dt)                                        */
+    int nsteps;      /* overlap-save steps of this trial                   */
+    int first_step;  /* index of its first step among the plan's steps     */
+} fdas_trial_info;
 
 typedef struct fdas_candidate {
-    int step;      /* step index                                         */
+    int step;      /* step index within its trial                        */
     int z_index;   /* fundamental row, z = -zmax + 2*z_index             */
     int r_index;   /* column, r = rlo + step*uselen/2 + 0.5*r_index      */
     int stage;     /* 0..4: summed over 1,2,4,8,16 harmonics             */
     float power;   /* summed normalised power                            */
-    float reserved;
+    int trial;     /* DM trial (0 in a single-series plan)               */
 } fdas_candidate;
 
 typedef struct fdas_plan fdas_plan;
@@ -208,10 +220,30 @@ int fdas_run_steps(fdas_plan *plan, int step0, int nsteps, float *d_planes);
 int fdas_search(fdas_plan *plan, const float *d_series);
 /* Copy of the whitened spectrum (nbins complex) to device memory d_out. */
 int fdas_get_spectrum(fdas_plan *plan, fdas_complex *d_out);
-/* Candidates found since fdas_prepare, in (step, z, r, stage) order. */
+/* Candidates found since the last reset, in (trial, step, z, r, stage) order. */
 int fdas_get_candidates(fdas_plan *plan, fdas_candidate *out, int capacity, int *count,
                         int *overflowed);
+int fdas_reset_candidates(fdas_plan *plan);
 int fdas_synchronize(fdas_plan *plan);
+
+/* Batches: many DM trials (any even lengths) searched with one configuration.
+ * Their steps are numbered consecutively (trial 0 first) and run in the same
+ * chunked launches.  The plan owns one device input buffer per trial.  Template
+ * banks are shared by all live plans of the same (zmax, numharm, uselen,
+ * precision) on a device. */
+int fdas_batch_create(const fdas_params *params, int ntrials, const long *lengths,
+                      fdas_plan **plan);   /* params->n is ignored */
+int fdas_batch_trial_info(const fdas_plan *plan, int trial, fdas_trial_info *info);
+/* Device input buffer of a trial (lengths[trial] floats); fill it before a run. */
+float *fdas_batch_series(fdas_plan *plan, int trial);
+/* R2C + whitening of one trial (d_series NULL: the plan's buffer). */
+int fdas_prepare_trial(fdas_plan *plan, int trial, const float *d_series);
+/* Every trial from the plan's buffers: prepare, reset candidates, all steps.
+ * With graphs enabled the first call records the work into a hipGraph and
+ * later calls replay it (the buffers' contents may change between calls). */
+int fdas_batch_run(fdas_plan *plan);
+int fdas_plan_enable_graph(fdas_plan *plan, int enable);
+int fdas_batch_spectrum(fdas_plan *plan, int trial, fdas_complex *d_out);
 
 /* Per-kernel timing with HIP events recorded on the plan's stream around each
  * launch (enable resets the counters; adds two event records per launch). */

@@ -1,0 +1,124 @@
+"""Batches of DM trials: one plan over many series of uneven lengths.
+
+A batch computes each (trial, step) item independently of what else shares its
+launch, so its planes and candidates must be bit-identical to single-series
+plans of the same trials, and those agree with the oracle to 1e-4 (relative to
+max(P_ref, 1)).  Graph replay must reproduce the eager run bit for bit, also
+after the inputs change between replays.
+"""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+POWER_TOL = 1e-4
+ZMAX, NUMHARM, FFTLEN = 50, 4, 2048
+LENGTHS = [1 << 18, 300_000, (1 << 18) + 2, 250_002, 1 << 19]
+POWCUT = (28.0, 36.0, 45.0, 0.0, 0.0)
+
+
+@pytest.fixture(scope="module")
+def fd():
+    import gpu_accelsearch_amd as fd
+    fd.lib()
+    return fd
+
+
+def _series(t, n, other=0):
+    from gpu_accelsearch_amd import synth
+    return synth.plain_series(n, seed=40 + t + 100 * other, r0=9000.7 + 1700.3 * t + 611.1 * other,
+                              z0=-20.0 + 9.0 * t - 3.0 * other, power=90.0)
+
+
+def _params(fd, n=0, chunk=0):
+    return fd.SearchParams(n=n, zmax=ZMAX, numharm=NUMHARM, fftlen=FFTLEN, chunk_steps=chunk,
+                           powcut=POWCUT)
+
+
+def _cand_set(c):
+    return {(int(a["trial"]), int(a["step"]), int(a["z_index"]), int(a["r_index"]),
+             int(a["stage"]), float(a["power"])) for a in c}
+
+
+def test_batch_matches_single_plans_and_oracle(fd, oracle):
+    xs = [_series(t, n) for t, n in enumerate(LENGTHS)]
+    print_case("batch uneven lengths", LENGTHS)
+    # chunk of 37 items: launches mix steps of different trials
+    with fd.BatchSearch(_params(fd, chunk=37), LENGTHS) as b:
+        assert b.info.ntrials == len(LENGTHS) and b.info.chunk_steps == 37
+        for t, x in enumerate(xs):
+            b.upload(t, x)
+        b.prepare()
+        firsts = [tr.first_step for tr in b.trials]
+        assert firsts[0] == 0
+        want_cands = set()
+        for t, (n, x) in enumerate(zip(LENGTHS, xs)):
+            tr = b.trials[t]
+            with fd.AccelSearch(_params(fd, n=n)) as s:
+                assert (tr.n, tr.nbins, tr.rlo, tr.nsteps) == (n, n // 2, s.rlo, s.nsteps)
+                s.prepare(x)
+                np.testing.assert_array_equal(b.spectrum(t).view(np.uint32),
+                                              s.spectrum().view(np.uint32))
+                single = s.run_steps(0, s.nsteps)
+                batch = b.run_steps(tr.first_step, tr.nsteps)
+                np.testing.assert_array_equal(batch.view(np.uint32), single.view(np.uint32))
+                s.search(x)
+                c = s.candidates()
+                c["trial"] = t
+                want_cands |= _cand_set(c)
+            ref = oracle.Search(x, n, 6.4e-5, ZMAX, NUMHARM, FFTLEN)
+            for st in (0, tr.nsteps // 2, tr.nsteps - 1):
+                want = ref.step_plane(st)
+                e = float(np.max(np.abs(batch[st] - want) / np.maximum(want, 1.0)))
+                assert e <= POWER_TOL, (t, st, e)
+            ref.close()
+        print_case("batch run + candidates")
+        b.run()
+        got = _cand_set(b.candidates())
+        assert not b.overflowed
+        assert len(got) > 0 and {g[0] for g in got} == set(range(len(LENGTHS)))
+        assert got == want_cands
+
+
+def test_batch_graph_replay(fd):
+    xs = [_series(t, n) for t, n in enumerate(LENGTHS)]
+    ys = [_series(t, n, other=1) for t, n in enumerate(LENGTHS)]
+    print_case("batch eager reference")
+    with fd.BatchSearch(_params(fd, chunk=64), LENGTHS) as e:
+        for t, x in enumerate(xs):
+            e.upload(t, x)
+        e.run()
+        cx = _cand_set(e.candidates())
+        for t, y in enumerate(ys):
+            e.upload(t, y)
+        e.run()
+        cy = _cand_set(e.candidates())
+        planes_y = e.run_steps(0, 20)
+    assert cx != cy and len(cx) > 0 and len(cy) > 0
+    print_case("batch graph replay")
+    with fd.BatchSearch(_params(fd, chunk=64), LENGTHS, graph=True) as g:
+        for rep, (data, want) in enumerate(((xs, cx), (xs, cx), (ys, cy), (xs, cx), (ys, cy))):
+            for t, x in enumerate(data):
+                g.upload(t, x)
+            g.run()
+            got = _cand_set(g.candidates())
+            assert got == want, rep
+            g.synchronize()
+        # the graph leaves the spectra of the last inputs behind
+        np.testing.assert_array_equal(g.run_steps(0, 20).view(np.uint32), planes_y.view(np.uint32))
+
+
+def test_template_banks_shared(fd):
+    print_case("shared template banks")
+    a = fd.AccelSearch(_params(fd, n=1 << 18))
+    b = fd.BatchSearch(_params(fd), [1 << 18, 1 << 19])
+    try:
+        assert a.info.template_bytes > 0
+        assert a.info.template_bytes == b.info.template_bytes
+        # plan-owned buffers do not include the shared banks
+        assert b.info.device_bytes > 0
+    finally:
+        a.close()
+        b.close()

@@ -49,6 +49,22 @@ def test_plan_rejects_bad_parameters(kw, code):
     assert fd.last_error()
 
 
+@pytest.mark.parametrize("lengths,code", [
+    ([], 1), ([1 << 20, 1001], 4), ([1 << 20, 1000], 1), ([1 << 20, 100], 4),
+    ([(1 << 27) + 2], 4),
+])
+def test_batch_rejects_bad_lengths(lengths, code):
+    """Every trial is checked on the host before any device state is made."""
+    import gpu_accelsearch_amd as fd
+    p = fd.SearchParams(n=0, zmax=200, numharm=8, fftlen=8192).to_c()
+    arr = (C.c_long * max(len(lengths), 1))(*lengths)
+    h = C.c_void_p()
+    rc = fd.lib().fdas_batch_create(C.byref(p), len(lengths), arr, C.byref(h))
+    assert rc == code, (lengths, rc, fd.last_error())
+    assert h.value is None
+    assert fd.last_error()
+
+
 def test_uselen_setting():
     import gpu_accelsearch_amd as fd
     L = fd.lib()
