@@ -6,14 +6,14 @@ first call that needs it does, and raises if it has not been built.
 """
 from ._native import FdasError, last_error, lib  # noqa: F401
 from .accel import (AccelSearch, BatchSearch, CANDIDATE_DTYPE, DeviceArray, SearchParams,  # noqa: F401
-                    StagePipeline, SubharmonicBank, calc_required_r, calc_required_z,
+                    Server, StagePipeline, SubharmonicBank, calc_required_r, calc_required_z,
                     calculateFFTLength, complexToComplexFFT, createSubharmonicInfo, deredden,
                     generate_harmonics_ffdot, getTotalHarmonics, index_from_r, index_from_z,
                     inverseComplexToComplexFFT, realToComplexFFT, stage_powcut, synchronize,
                     z_response_halfwidth)
 
 __all__ = [
-    "AccelSearch", "BatchSearch", "SearchParams", "DeviceArray", "StagePipeline", "SubharmonicBank",
+    "AccelSearch", "BatchSearch", "Server", "SearchParams", "DeviceArray", "StagePipeline", "SubharmonicBank",
     "createSubharmonicInfo", "realToComplexFFT", "complexToComplexFFT",
     "inverseComplexToComplexFFT", "deredden", "generate_harmonics_ffdot", "calc_required_z",
     "calc_required_r", "index_from_r", "index_from_z", "getTotalHarmonics",

@@ -71,6 +71,28 @@ class FdasCandidate(C.Structure):
                 ("stage", C.c_int), ("power", C.c_float), ("trial", C.c_int)]
 
 
+class FdasServerConfig(C.Structure):
+    _fields_ = [("memory_bytes", C.c_size_t), ("max_active", C.c_int), ("chunk_steps", C.c_int),
+                ("max_candidates", C.c_int)]
+
+
+class FdasRequestResult(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("ncandidates", C.c_int),
+                ("overflowed", C.c_int), ("nsteps", C.c_int), ("cells", C.c_longlong),
+                ("device_bytes", C.c_size_t), ("queue_ms", C.c_double),
+                ("service_ms", C.c_double), ("latency_ms", C.c_double),
+                ("message", C.c_char * 256)]
+
+
+class FdasServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_longlong), ("completed", C.c_longlong),
+                ("failed", C.c_longlong), ("active", C.c_int), ("pending", C.c_int),
+                ("max_active_seen", C.c_int), ("memory_bytes", C.c_size_t),
+                ("memory_in_use", C.c_size_t), ("memory_peak", C.c_size_t),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("busy_s", C.c_double), ("trials_per_s", C.c_double)]
+
+
 vp = C.c_void_p
 i32 = C.c_int
 u16 = C.c_ushort
@@ -131,6 +153,14 @@ PROTOTYPES = [
     ("fdas_batch_spectrum", i32, [vp, i32, vp]),
     ("fdas_plan_enable_timing", i32, [vp, i32]),
     ("fdas_plan_get_timing", i32, [vp, C.POINTER(FdasTiming)]),
+    ("fdas_server_create", i32, [C.POINTER(FdasServerConfig), C.POINTER(vp)]),
+    ("fdas_server_destroy", None, [vp]),
+    ("fdas_server_prewarm", i32, [vp, C.POINTER(FdasParams)]),
+    ("fdas_server_submit", i32, [vp, C.POINTER(FdasParams), vp, C.POINTER(C.c_longlong)]),
+    ("fdas_server_wait", i32, [vp, C.c_longlong, C.POINTER(FdasRequestResult),
+                               C.POINTER(FdasCandidate), i32]),
+    ("fdas_server_get_stats", i32, [vp, C.POINTER(FdasServerStats)]),
+    ("fdas_server_reset_stats", i32, [vp]),
     ("fdas_malloc", i32, [C.POINTER(vp), C.c_size_t]),
     ("fdas_free", i32, [vp]),
     ("fdas_memcpy_htod", i32, [vp, vp, C.c_size_t]),

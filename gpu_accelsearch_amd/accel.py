@@ -295,6 +295,66 @@ class BatchSearch:
     __del__ = AccelSearch.__del__
 
 
+class Server:
+    """Continuous-batching request loop over the plan API (fdas_server_*).
+
+    ``submit`` copies a host series and returns at once; a dispatcher thread in
+    the library admits requests while a slot and device memory are free and runs
+    them concurrently on separate streams.  ``wait`` returns the request's
+    result record and candidates."""
+
+    def __init__(self, memory_bytes: int = 0, max_active: int = 4, chunk_steps: int = 0,
+                 max_candidates: int = 0):
+        cfg = N.FdasServerConfig(memory_bytes, max_active, chunk_steps, max_candidates)
+        h = C.c_void_p()
+        check(lib().fdas_server_create(C.byref(cfg), C.byref(h)), "fdas_server_create")
+        self.handle = h.value
+
+    def prewarm(self, params: SearchParams) -> None:
+        cp = params.to_c()
+        check(lib().fdas_server_prewarm(self.handle, C.byref(cp)), "fdas_server_prewarm")
+
+    def submit(self, params: SearchParams, series: np.ndarray) -> int:
+        x = np.ascontiguousarray(series, dtype=np.float32)
+        if params.n != len(x):
+            params = SearchParams(**{**params.__dict__, "n": len(x)})
+        cp = params.to_c()
+        rid = C.c_longlong()
+        check(lib().fdas_server_submit(self.handle, C.byref(cp), x.ctypes.data, C.byref(rid)),
+              "fdas_server_submit")
+        return rid.value
+
+    def wait(self, rid: int, capacity: int = 1 << 20):
+        """(result dict, candidates) of request ``rid``; raises if it failed."""
+        res = N.FdasRequestResult()
+        out = np.zeros(capacity, CANDIDATE_DTYPE)
+        check(lib().fdas_server_wait(self.handle, rid, C.byref(res),
+                                     out.ctypes.data_as(C.POINTER(N.FdasCandidate)), capacity),
+              "fdas_server_wait")
+        info = {k: getattr(res, k) for k, _ in N.FdasRequestResult._fields_ if k != "message"}
+        if res.status != 0:
+            raise N.FdasError(f"request {rid}: {N.ERRORS.get(res.status, res.status)}: "
+                              f"{res.message.decode()}")
+        return info, out[:min(res.ncandidates, capacity)].copy()
+
+    def stats(self) -> dict:
+        st = N.FdasServerStats()
+        check(lib().fdas_server_get_stats(self.handle, C.byref(st)), "fdas_server_get_stats")
+        return {k: getattr(st, k) for k, _ in N.FdasServerStats._fields_}
+
+    def reset_stats(self) -> None:
+        check(lib().fdas_server_reset_stats(self.handle), "fdas_server_reset_stats")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            lib().fdas_server_destroy(self.handle)
+            self.handle = None
+
+    __enter__ = AccelSearch.__enter__
+    __exit__ = AccelSearch.__exit__
+    __del__ = AccelSearch.__del__
+
+
 def stage_powcut(sigma: float, numharm: int, numindep: float = 1.0) -> float:
     """Summed power whose false-alarm probability over ``numindep`` trials equals
     the Gaussian tail beyond ``sigma``.  Each normalised power is Exp(1) under

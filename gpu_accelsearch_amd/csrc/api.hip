@@ -32,6 +32,30 @@ void set_error(const char* fmt, ...)
 
 const char* last_error() { return g_err; }
 
+namespace {
+struct HipAlloc final : DevAlloc {
+    int alloc(void** p, size_t bytes) override
+    {
+        const hipError_t e = hipMalloc(p, bytes > 0 ? bytes : 1);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return hip_fail(e, "hipMalloc", __FILE__, __LINE__);
+        }
+        return FDAS_OK;
+    }
+    void release(void* p) override
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+DevAlloc* hip_alloc()
+{
+    static HipAlloc a;
+    return &a;
+}
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line)
 {
     set_error("%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
@@ -190,7 +214,7 @@ extern "C" unsigned realToComplexFFT(float* idata, long n, fdas_complex** odata,
     }
     *odata = nullptr;
     R2CPlan rp;
-    if (r2c_plan_create(n, &rp, 0) != FDAS_OK) return 0;
+    if (r2c_plan_create(n, &rp) != FDAS_OK) return 0;
     const long nout = n / 2 + 1;
     float* d_in = nullptr;
     cplx *d_out = nullptr, *work = nullptr;
@@ -214,7 +238,6 @@ extern "C" unsigned realToComplexFFT(float* idata, long n, fdas_complex** odata,
     (void)hipFree(d_in);
     (void)hipFree(d_out);
     (void)hipFree(work);
-    r2c_plan_destroy(&rp);
     if (!ret) {
         free(host);
         host = nullptr;

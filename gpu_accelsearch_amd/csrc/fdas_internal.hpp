@@ -51,15 +51,24 @@ int fft_c2c(const cplx* in, cplx* out, long m, int dir, cplx* work, hipStream_t 
 // Real-to-complex transform of n (even) floats -> n/2+1 bins.  The n/2-point
 // complex transform of the (even, odd) pairs is a four-step FFT when n/2 is a
 // power of two, else a chirp-z (Bluestein) transform through length p >= n-1.
+// A plan is geometry only (no device state); work holds work_elems complex.
 struct R2CPlan {
     long n = 0, m = 0, p = 0;
     bool pow2 = true;
     size_t work_elems = 0;     // complex scratch the transform needs
-    cplx* d_kernel = nullptr;  // FFT of the chirp (Bluestein only)
 };
-int r2c_plan_create(long n, R2CPlan* rp, hipStream_t s);
-void r2c_plan_destroy(R2CPlan* rp);
+int r2c_plan_create(long n, R2CPlan* rp);
 int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipStream_t s);
+
+// ---------------------------------------------------------------- device memory
+// Source of a plan's device buffers: hipMalloc/hipFree, or a sub-allocator over
+// one reserved region (serving, where hipFree would wait for the whole device).
+struct DevAlloc {
+    virtual ~DevAlloc() = default;
+    virtual int alloc(void** p, size_t bytes) = 0;
+    virtual void release(void* p) = 0;
+};
+DevAlloc* hip_alloc();
 
 // ---------------------------------------------------------------- whitening
 struct WhitenSchedule {
@@ -72,8 +81,10 @@ struct WhitenSchedule {
     int* d_seg_lo = nullptr;    // first bin of segment
     int* d_seg_hi = nullptr;    // one past the last bin
 };
-int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws);
-void whiten_schedule_destroy(WhitenSchedule* ws);
+// Tables are copied on stream s, which is synchronised before returning.
+int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws, DevAlloc* al = nullptr,
+                           hipStream_t s = 0);
+void whiten_schedule_destroy(WhitenSchedule* ws, DevAlloc* al = nullptr);
 int whiten(cplx* d_spec, const WhitenSchedule& ws, hipStream_t s);
 
 // ---------------------------------------------------------------- templates
@@ -163,6 +174,11 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
 int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s);
+
+// Plan creation on a given stream (0: the plan makes its own) with buffers from
+// al (nullptr: hipMalloc).  Implemented in plan.hip.
+int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
+                   hipStream_t stream, DevAlloc* al, fdas_plan** out);
 
 // Stage-API kernels (unfused, reference data layout).
 int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,

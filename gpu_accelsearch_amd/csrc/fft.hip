@@ -58,6 +58,7 @@ static std::vector<cplx> host_twiddles_for(int n, int t)
     if (t != fft_threads(n)) {
         if (n == 8192 && t == 512) return host_twiddles<8192, 512>();
         if (n == 4096 && t == 512) return host_twiddles<4096, 512>();
+        if (n == 128 && t == 64) return host_twiddles<128, 64>();
         return {};
     }
     switch (n) {
@@ -299,7 +300,7 @@ __global__ __launch_bounds__(256) void bluestein_post_kernel(const cplx* __restr
 
 static unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
 
-int r2c_plan_create(long n, R2CPlan* rp, hipStream_t s)
+int r2c_plan_create(long n, R2CPlan* rp)
 {
     *rp = R2CPlan();
     if (n < 256 || (n & 1)) {
@@ -320,24 +321,9 @@ int r2c_plan_create(long n, R2CPlan* rp, hipStream_t s)
         set_error("real transform length %ld too long for the chirp-z path", n);
         return FDAS_ERR_UNSUPPORTED;
     }
-    // Z (m) + a (p) + four-step scratch (2p)
-    rp->work_elems = (size_t)rp->m + 3 * (size_t)rp->p;
-    FDAS_HIP(hipMalloc(&rp->d_kernel, (size_t)rp->p * sizeof(cplx)));
-    cplx* scratch = nullptr;
-    FDAS_HIP(hipMalloc(&scratch, 2 * (size_t)rp->p * sizeof(cplx)));
-    hipLaunchKernelGGL(bluestein_kernel_kernel, dim3(blocks_for(rp->p)), dim3(256), 0, s,
-                       rp->d_kernel, rp->m, rp->p);
-    int rc = check_launch("bluestein_kernel_kernel");
-    if (rc == FDAS_OK) rc = fft_c2c(rp->d_kernel, rp->d_kernel, rp->p, -1, scratch, s);
-    if (hipStreamSynchronize(s) != hipSuccess && rc == FDAS_OK) rc = FDAS_ERR_HIP;
-    (void)hipFree(scratch);
-    return rc;
-}
-
-void r2c_plan_destroy(R2CPlan* rp)
-{
-    (void)hipFree(rp->d_kernel);
-    *rp = R2CPlan();
+    // Z (m) + a (p) + transformed chirp (p) + four-step scratch (2p)
+    rp->work_elems = (size_t)rp->m + 4 * (size_t)rp->p;
+    return FDAS_OK;
 }
 
 int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipStream_t s)
@@ -352,14 +338,20 @@ int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipS
     if (rp.pow2) {
         FDAS_TRY(fft_c2c(z, Z, m, -1, work + m, s));
     } else {
+        // chirp-z: a = x w, A = FFT(a) FFT(b) / p, Z = conj(w) IFFT(A); the
+        // transformed chirp b is rebuilt per call, so a plan holds no device state
         const long p = rp.p;
         cplx* a = work + m;
-        cplx* scratch = a + p;
+        cplx* kb = a + p;
+        cplx* scratch = kb + p;
+        hipLaunchKernelGGL(bluestein_kernel_kernel, dim3(blocks_for(p)), dim3(256), 0, s, kb, m, p);
+        FDAS_TRY(check_launch("bluestein_kernel_kernel"));
+        FDAS_TRY(fft_c2c(kb, kb, p, -1, scratch, s));
         hipLaunchKernelGGL(bluestein_pre_kernel, dim3(blocks_for(p)), dim3(256), 0, s, z, a, m, p);
         FDAS_TRY(check_launch("bluestein_pre_kernel"));
         FDAS_TRY(fft_c2c(a, a, p, -1, scratch, s));
-        hipLaunchKernelGGL(pointwise_mul_kernel, dim3(blocks_for(p)), dim3(256), 0, s, a,
-                           rp.d_kernel, p, (float)(1.0 / (double)p));
+        hipLaunchKernelGGL(pointwise_mul_kernel, dim3(blocks_for(p)), dim3(256), 0, s, a, kb, p,
+                           (float)(1.0 / (double)p));
         FDAS_TRY(check_launch("pointwise_mul_kernel"));
         FDAS_TRY(fft_c2c(a, a, p, +1, scratch, s));
         hipLaunchKernelGGL(bluestein_post_kernel, dim3(blocks_for(m)), dim3(256), 0, s, a, Z, m);

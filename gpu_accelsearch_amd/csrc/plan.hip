@@ -149,6 +149,7 @@ struct fdas_plan {
     HarmSumArgs hs{};
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    DevAlloc* al = nullptr;         // source of the plan-owned buffers
     size_t bytes = 0;
     // hipGraph of a whole batch run
     bool use_graph = false;
@@ -189,8 +190,14 @@ struct TimedLaunch {
 
 int plan_alloc(fdas_plan* pl, void** ptr, size_t bytes)
 {
-    FDAS_HIP(hipMalloc(ptr, bytes > 0 ? bytes : 1));
+    FDAS_TRY(pl->al->alloc(ptr, bytes));
     pl->bytes += bytes;
+    return FDAS_OK;
+}
+
+int plan_upload(fdas_plan* pl, void* dst, const void* src, size_t bytes)
+{
+    FDAS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, pl->stream));
     return FDAS_OK;
 }
 
@@ -388,12 +395,26 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
     return FDAS_OK;
 }
 
+int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
+                const std::vector<int>& rlo_h, const std::vector<std::vector<SegDesc>>& desc);
+
+// Host tables outlive every copy queued from them: the stream is synchronised
+// before they go out of scope, on success and on failure.
 int plan_build(fdas_plan* pl, const long* lengths, int ntrials, bool own_series)
 {
     FDAS_TRY(plan_geometry(pl, lengths, ntrials));
     std::vector<int> fullrlo, rlo_h;
     std::vector<std::vector<SegDesc>> desc;
     FDAS_TRY(plan_tables(pl, fullrlo, rlo_h, desc));
+    const int rc = plan_device(pl, own_series, fullrlo, rlo_h, desc);
+    if (pl->stream && hipStreamSynchronize(pl->stream) != hipSuccess && rc == FDAS_OK)
+        return hip_fail(hipGetLastError(), "hipStreamSynchronize", __FILE__, __LINE__);
+    return rc;
+}
+
+int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
+                const std::vector<int>& rlo_h, const std::vector<std::vector<SegDesc>>& desc)
+{
     const fdas_params& p = pl->p;
     const int U = pl->uselen, H = pl->H;
 
@@ -405,19 +426,21 @@ int plan_build(fdas_plan* pl, const long* lengths, int ntrials, bool own_series)
     pl->chunk = std::min(chunk, pl->nitems);
 
     // device state (everything above is host-side validation and geometry)
-    FDAS_HIP(hipStreamCreate(&pl->stream));
-    pl->own_stream = true;
+    if (!pl->stream) {
+        FDAS_HIP(hipStreamCreate(&pl->stream));
+        pl->own_stream = true;
+    }
     FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, U, pl->prec, pl->stream, &pl->banks));
     for (size_t c = 0; c < desc.size(); ++c) {
         SegDesc* dd = nullptr;
         FDAS_TRY(plan_alloc(pl, (void**)&dd, desc[c].size() * sizeof(SegDesc)));
         pl->d_desc.push_back(dd);
-        FDAS_HIP(hipMemcpy(dd, desc[c].data(), desc[c].size() * sizeof(SegDesc), hipMemcpyHostToDevice));
+        FDAS_TRY(plan_upload(pl, dd, desc[c].data(), desc[c].size() * sizeof(SegDesc)));
     }
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_fullrlo, fullrlo.size() * sizeof(int)));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_rlo_h, rlo_h.size() * sizeof(int)));
-    FDAS_HIP(hipMemcpy(pl->d_fullrlo, fullrlo.data(), fullrlo.size() * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_HIP(hipMemcpy(pl->d_rlo_h, rlo_h.data(), rlo_h.size() * sizeof(int), hipMemcpyHostToDevice));
+    FDAS_TRY(plan_upload(pl, pl->d_fullrlo, fullrlo.data(), fullrlo.size() * sizeof(int)));
+    FDAS_TRY(plan_upload(pl, pl->d_rlo_h, rlo_h.data(), rlo_h.size() * sizeof(int)));
     long long spec = 0, series = 0;
     size_t work = 0;
     for (TrialInfo& tr : pl->trials) {
@@ -428,8 +451,9 @@ int plan_build(fdas_plan* pl, const long* lengths, int ntrials, bool own_series)
         if (f == pl->forms.size()) {
             pl->forms.emplace_back();
             pl->forms.back().n = tr.n;
-            FDAS_TRY(r2c_plan_create(tr.n, &pl->forms.back().r2c, pl->stream));
-            FDAS_TRY(whiten_schedule_create((unsigned)tr.nbins, &pl->forms.back().ws));
+            FDAS_TRY(r2c_plan_create(tr.n, &pl->forms.back().r2c));
+            FDAS_TRY(whiten_schedule_create((unsigned)tr.nbins, &pl->forms.back().ws, pl->al,
+                                            pl->stream));
             work = std::max(work, pl->forms.back().r2c.work_elems);
         }
         tr.form = (int)f;
@@ -557,8 +581,11 @@ int batch_body(fdas_plan* pl)
     return run_items(pl, 0, pl->nitems, nullptr);
 }
 
-int create_plan(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
-                fdas_plan** out)
+}  // namespace
+
+namespace fdas {
+int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
+                   hipStream_t stream, DevAlloc* al, fdas_plan** out)
 {
     if (!params || !out || ntrials < 1 || !lengths) {
         set_error("fdas plan: null argument or no trials");
@@ -567,6 +594,8 @@ int create_plan(const fdas_params* params, int ntrials, const long* lengths, boo
     *out = nullptr;
     fdas_plan* pl = new fdas_plan();
     pl->p = *params;
+    pl->al = al ? al : hip_alloc();
+    pl->stream = stream;
     const int rc = plan_build(pl, lengths, ntrials, own_series);
     if (rc != FDAS_OK) {
         fdas_plan_destroy(pl);
@@ -576,7 +605,10 @@ int create_plan(const fdas_params* params, int ntrials, const long* lengths, boo
     return FDAS_OK;
 }
 
-}  // namespace
+// Shared handle on a plan's template banks (keeps them resident while held).
+std::shared_ptr<void> plan_banks(fdas_plan* pl) { return pl->banks; }
+
+}  // namespace fdas
 
 // ====================================================================== C ABI
 extern "C" void fdas_plan_destroy(fdas_plan* pl)
@@ -584,22 +616,20 @@ extern "C" void fdas_plan_destroy(fdas_plan* pl)
     if (!pl) return;
     if (pl->stream) (void)hipStreamSynchronize(pl->stream);
     if (pl->graph) (void)hipGraphExecDestroy(pl->graph);
-    for (SegDesc* d : pl->d_desc) (void)hipFree(d);
-    (void)hipFree(pl->d_fullrlo);
-    (void)hipFree(pl->d_rlo_h);
-    (void)hipFree(pl->d_series);
-    (void)hipFree(pl->d_spec);
-    (void)hipFree(pl->d_r2c_work);
-    for (LengthForm& lf : pl->forms) {
-        r2c_plan_destroy(&lf.r2c);
-        whiten_schedule_destroy(&lf.ws);
-    }
-    (void)hipFree(pl->d_seg);
-    (void)hipFree(pl->d_seg_exp);
-    (void)hipFree(pl->d_sub);
-    (void)hipFree(pl->d_plane);
-    (void)hipFree(pl->d_cands);
-    (void)hipFree(pl->d_ncand);
+    DevAlloc* al = pl->al ? pl->al : hip_alloc();
+    for (SegDesc* d : pl->d_desc) al->release(d);
+    al->release(pl->d_fullrlo);
+    al->release(pl->d_rlo_h);
+    al->release(pl->d_series);
+    al->release(pl->d_spec);
+    al->release(pl->d_r2c_work);
+    for (LengthForm& lf : pl->forms) whiten_schedule_destroy(&lf.ws, al);
+    al->release(pl->d_seg);
+    al->release(pl->d_seg_exp);
+    al->release(pl->d_sub);
+    al->release(pl->d_plane);
+    al->release(pl->d_cands);
+    al->release(pl->d_ncand);
     for (auto& r : pl->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -616,13 +646,13 @@ extern "C" int fdas_plan_create(const fdas_params* params, fdas_plan** out)
         return FDAS_ERR_INVALID;
     }
     const long n = params->n;
-    return create_plan(params, 1, &n, false, out);
+    return plan_create_ex(params, 1, &n, false, nullptr, nullptr, out);
 }
 
 extern "C" int fdas_batch_create(const fdas_params* params, int ntrials, const long* lengths,
                                  fdas_plan** out)
 {
-    return create_plan(params, ntrials, lengths, true, out);
+    return plan_create_ex(params, ntrials, lengths, true, nullptr, nullptr, out);
 }
 
 extern "C" int fdas_plan_get_info(const fdas_plan* pl, fdas_plan_info* info)
@@ -821,15 +851,19 @@ extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capac
         set_error("fdas_get_candidates: null argument");
         return FDAS_ERR_INVALID;
     }
-    FDAS_HIP(hipStreamSynchronize(pl->stream));
     int n = 0;
-    FDAS_HIP(hipMemcpy(&n, pl->d_ncand, sizeof(int), hipMemcpyDeviceToHost));
+    FDAS_HIP(hipMemcpyAsync(&n, pl->d_ncand, sizeof(int), hipMemcpyDeviceToHost, pl->stream));
+    FDAS_HIP(hipStreamSynchronize(pl->stream));
     const int kept = std::min(n, pl->cand_cap);
     if (overflowed) *overflowed = n > pl->cand_cap;
+    *count = kept;
+    if (!out || capacity <= 0) return FDAS_OK;
     std::vector<fdas_candidate> all(kept);
-    if (kept > 0)
-        FDAS_HIP(hipMemcpy(all.data(), pl->d_cands, (size_t)kept * sizeof(fdas_candidate),
-                           hipMemcpyDeviceToHost));
+    if (kept > 0) {
+        FDAS_HIP(hipMemcpyAsync(all.data(), pl->d_cands, (size_t)kept * sizeof(fdas_candidate),
+                                hipMemcpyDeviceToHost, pl->stream));
+        FDAS_HIP(hipStreamSynchronize(pl->stream));
+    }
     // kernels record the item; report (trial, step within the trial)
     for (auto& c : all) {
         int t = (int)pl->trials.size() - 1;
@@ -844,8 +878,7 @@ extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capac
         if (a.r_index != b.r_index) return a.r_index < b.r_index;
         return a.stage < b.stage;
     });
-    *count = kept;
-    if (out && capacity > 0) memcpy(out, all.data(), sizeof(fdas_candidate) * std::min(kept, capacity));
+    memcpy(out, all.data(), sizeof(fdas_candidate) * std::min(kept, capacity));
     return FDAS_OK;
 }
 

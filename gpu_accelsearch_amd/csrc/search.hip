@@ -280,12 +280,8 @@ template <int NH, int P>
 static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
 {
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles(NH, &err);
+    const cplx* tw = fft_twiddles_t(NH, FftCfgS<NH>::T, &err);
     if (!tw) return err;
-    if (fft_threads(NH) != FftCfgS<NH>::T) {
-        set_error("thread count mismatch for length %d", NH);
-        return FDAS_ERR_UNSUPPORTED;
-    }
     hipLaunchKernelGGL((seg_prep_kernel<NH, P>), dim3(a.nseg), dim3(FftCfgS<NH>::T), 0, s, a.spec,
                        a.desc, static_cast<typename CStore<P>::T*>(a.seg), a.out_base,
                        a.seg_exp, a.eslot_base, tw);

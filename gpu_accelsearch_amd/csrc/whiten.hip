@@ -132,8 +132,9 @@ __global__ __launch_bounds__(256) void whiten_norm_kernel(cplx* __restrict__ spe
 }
 
 // Host: the block and segment schedule of source/bpd_noise.c:33-88.
-int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws)
+int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws, DevAlloc* al, hipStream_t s)
 {
+    if (!al) al = hip_alloc();
     if (nbins < 16) {
         set_error("whitening needs at least 16 bins (got %u)", nbins);
         return FDAS_ERR_INVALID;
@@ -163,25 +164,27 @@ int whiten_schedule_create(unsigned nbins, WhitenSchedule* ws)
     ws->nblocks = (int)start.size();
     ws->nsegs = (int)lo.size();
     const size_t nb = start.size(), ns = lo.size();
-    FDAS_HIP(hipMalloc(&ws->d_start, nb * sizeof(int)));
-    FDAS_HIP(hipMalloc(&ws->d_len, nb * sizeof(int)));
-    FDAS_HIP(hipMalloc(&ws->d_median, nb * sizeof(float)));
-    FDAS_HIP(hipMalloc(&ws->d_seg_lo, ns * sizeof(int)));
-    FDAS_HIP(hipMalloc(&ws->d_seg_hi, ns * sizeof(int)));
-    FDAS_HIP(hipMemcpy(ws->d_start, start.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_HIP(hipMemcpy(ws->d_len, len.data(), nb * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_HIP(hipMemcpy(ws->d_seg_lo, lo.data(), ns * sizeof(int), hipMemcpyHostToDevice));
-    FDAS_HIP(hipMemcpy(ws->d_seg_hi, hi.data(), ns * sizeof(int), hipMemcpyHostToDevice));
+    FDAS_TRY(al->alloc((void**)&ws->d_start, nb * sizeof(int)));
+    FDAS_TRY(al->alloc((void**)&ws->d_len, nb * sizeof(int)));
+    FDAS_TRY(al->alloc((void**)&ws->d_median, nb * sizeof(float)));
+    FDAS_TRY(al->alloc((void**)&ws->d_seg_lo, ns * sizeof(int)));
+    FDAS_TRY(al->alloc((void**)&ws->d_seg_hi, ns * sizeof(int)));
+    FDAS_HIP(hipMemcpyAsync(ws->d_start, start.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    FDAS_HIP(hipMemcpyAsync(ws->d_len, len.data(), nb * sizeof(int), hipMemcpyHostToDevice, s));
+    FDAS_HIP(hipMemcpyAsync(ws->d_seg_lo, lo.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    FDAS_HIP(hipMemcpyAsync(ws->d_seg_hi, hi.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    FDAS_HIP(hipStreamSynchronize(s));
     return FDAS_OK;
 }
 
-void whiten_schedule_destroy(WhitenSchedule* ws)
+void whiten_schedule_destroy(WhitenSchedule* ws, DevAlloc* al)
 {
-    (void)hipFree(ws->d_start);
-    (void)hipFree(ws->d_len);
-    (void)hipFree(ws->d_median);
-    (void)hipFree(ws->d_seg_lo);
-    (void)hipFree(ws->d_seg_hi);
+    if (!al) al = hip_alloc();
+    al->release(ws->d_start);
+    al->release(ws->d_len);
+    al->release(ws->d_median);
+    al->release(ws->d_seg_lo);
+    al->release(ws->d_seg_hi);
     *ws = WhitenSchedule();
 }
 

@@ -264,6 +264,63 @@ int fdas_plan_enable_timing(fdas_plan *plan, int enable);
 int fdas_plan_get_timing(fdas_plan *plan, fdas_timing *out);
 
 /* ------------------------------------------------------------------------ */
+/* 3. Serving: a continuous-batching request loop                            */
+/* ------------------------------------------------------------------------ */
+/* Requests (one DM-trial series each, any zmax / numharm / fftlen / precision)
+ * are admitted in arrival order while one of max_active slots is free and the
+ * request's plan (its f-fdot planes for chunk_steps steps, spectrum, segments,
+ * candidates) fits in the server's reserved device memory; admitted requests
+ * run concurrently on their own streams.  Templates stay resident per
+ * configuration.  A request's results do not depend on what runs beside it. */
+typedef struct fdas_server fdas_server;
+
+typedef struct fdas_server_config {
+    size_t memory_bytes;   /* device memory reserved for requests (0: half of free) */
+    int max_active;        /* requests on the device at once, 1..16 (0: 4)          */
+    int chunk_steps;       /* default steps per launch group (0: 256)               */
+    int max_candidates;    /* default candidate capacity per request (0: 65536)     */
+} fdas_server_config;
+
+typedef struct fdas_request_result {
+    long long id;
+    int status;            /* FDAS_OK or the error that ended the request        */
+    int ncandidates;
+    int overflowed;
+    int nsteps;
+    long long cells;
+    size_t device_bytes;   /* footprint admitted against the memory budget        */
+    double queue_ms;       /* submit -> admitted                                  */
+    double service_ms;     /* admitted -> complete on the device                  */
+    double latency_ms;     /* submit -> complete                                  */
+    char message[256];     /* error text when status != FDAS_OK                   */
+} fdas_request_result;
+
+typedef struct fdas_server_stats {
+    long long submitted, completed, failed;
+    int active, pending, max_active_seen;
+    size_t memory_bytes, memory_in_use, memory_peak;
+    double p50_ms, p99_ms, mean_ms;   /* latency of completed requests           */
+    double busy_s;                    /* time with at least one request active   */
+    double trials_per_s;              /* completed / (last completion - first submit) */
+} fdas_server_stats;
+
+int fdas_server_create(const fdas_server_config *cfg, fdas_server **srv);
+/* Finishes queued and running requests, then frees everything. */
+void fdas_server_destroy(fdas_server *srv);
+/* Builds (or finds) the template banks of a configuration and keeps them. */
+int fdas_server_prewarm(fdas_server *srv, const fdas_params *params);
+/* Queues a copy of h_series (params->n floats); returns at once with its id.
+ * params->chunk_steps / max_candidates of 0 take the server's defaults. */
+int fdas_server_submit(fdas_server *srv, const fdas_params *params, const float *h_series,
+                       long long *id);
+/* Blocks until request id is done; fills res and up to capacity candidates
+ * (trial 0, step within the series), then forgets the request. */
+int fdas_server_wait(fdas_server *srv, long long id, fdas_request_result *res,
+                     fdas_candidate *cands, int capacity);
+int fdas_server_get_stats(fdas_server *srv, fdas_server_stats *out);
+int fdas_server_reset_stats(fdas_server *srv);
+
+/* ------------------------------------------------------------------------ */
 /* Memory plumbing and diagnostics                                           */
 /* ------------------------------------------------------------------------ */
 int fdas_malloc(void **d_ptr, size_t bytes);

@@ -44,20 +44,25 @@ static void sim(const float* in, float* out)
 }
 
 // The (N, T) shapes the library instantiates.
-extern "C" int fft_sim(int n, int dir, const float* in, float* out)
+extern "C" int fft_sim(int n, int t, int dir, const float* in, float* out)
 {
 #define CASE(NN, TT)                                          \
-    if (n == NN) {                                            \
+    if (n == NN && t == TT) {                                 \
         if (dir < 0) sim<NN, TT, -1>(in, out);                \
         else sim<NN, TT, +1>(in, out);                        \
         return 0;                                             \
     }
+    CASE(64, 16)
+    CASE(128, 32)
+    CASE(128, 64)
     CASE(256, 64)
     CASE(512, 64)
     CASE(1024, 64)
     CASE(2048, 128)
     CASE(4096, 256)
+    CASE(4096, 512)
     CASE(8192, 256)
+    CASE(8192, 512)
     CASE(16384, 512)
 #undef CASE
     return -1;

@@ -152,7 +152,8 @@ def test_plan_harmonics_4_16(fd, oracle):
     from gpu_accelsearch_amd import synth
     n = 1 << 18
     x = synth.plain_series(n, seed=5, r0=9000.7, z0=-10.0, power=60.0)
-    for numharm, fftlen in ((2, 1024), (4, 2048), (16, 4096)):
+    # (16, 2048): the 1/16 harmonic runs at FFT length 256 (128-point segments)
+    for numharm, fftlen in ((2, 1024), (4, 2048), (16, 4096), (16, 2048)):
         print_case("plan numharm", numharm)
         _plan_vs_oracle(fd, oracle, x, n, 60, numharm, fftlen, [0, 3, 7])
 

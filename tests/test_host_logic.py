@@ -141,13 +141,16 @@ def fft_sim(tmp_path_factory):
     return C.CDLL(so)
 
 
-@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096, 8192, 16384])
-def test_workgroup_fft_index_arithmetic(fft_sim, n):
+@pytest.mark.parametrize("n,t", [(64, 16), (128, 32), (128, 64), (256, 64), (512, 64),
+                                 (1024, 64), (2048, 128), (4096, 256), (4096, 512), (8192, 256),
+                                 (8192, 512), (16384, 512)])
+def test_workgroup_fft_index_arithmetic(fft_sim, n, t):
+    """Every (length, threads) shape the library instantiates."""
     rng = np.random.default_rng(n)
     for sign in (-1, 1):
         x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
         out = np.zeros(n, np.complex64)
-        rc = fft_sim.fft_sim(n, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
+        rc = fft_sim.fft_sim(n, t, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
         assert rc == 0
         ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
         assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
