@@ -5,7 +5,7 @@ ABI in include/fdas.h.  Importing the package does not load the library; the
 first call that needs it does, and raises if it has not been built.
 """
 from ._native import FdasError, last_error, lib  # noqa: F401
-from .accel import (AccelSearch, BatchSearch, CANDIDATE_DTYPE, DeviceArray, SearchParams,  # noqa: F401
+from .accel import (AccelSearch, BatchSearch, autotune, resolve_fftlen, CANDIDATE_DTYPE, DeviceArray, SearchParams,  # noqa: F401
                     Server, StagePipeline, SubharmonicBank, calc_required_r, calc_required_z,
                     calculateFFTLength, complexToComplexFFT, createSubharmonicInfo, deredden,
                     generate_harmonics_ffdot, getTotalHarmonics, index_from_r, index_from_z,
@@ -18,5 +18,5 @@ __all__ = [
     "inverseComplexToComplexFFT", "deredden", "generate_harmonics_ffdot", "calc_required_z",
     "calc_required_r", "index_from_r", "index_from_z", "getTotalHarmonics",
     "z_response_halfwidth", "calculateFFTLength", "stage_powcut", "synchronize", "lib",
-    "FdasError", "last_error", "CANDIDATE_DTYPE",
+    "FdasError", "last_error", "CANDIDATE_DTYPE", "autotune", "resolve_fftlen",
 ]

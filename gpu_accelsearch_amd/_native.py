@@ -93,6 +93,17 @@ class FdasServerStats(C.Structure):
                 ("busy_s", C.c_double), ("trials_per_s", C.c_double)]
 
 
+TUNE_MAX = 8
+
+
+class FdasTuneReport(C.Structure):
+    _fields_ = [("fftlen", C.c_int), ("ncand", C.c_int),
+                ("cand_fftlen", C.c_int * TUNE_MAX), ("cand_ms_per_trial", C.c_double * TUNE_MAX),
+                ("ncorr", C.c_int), ("corr_len", C.c_int * TUNE_MAX),
+                ("corr_variant", C.c_int * TUNE_MAX), ("corr_threads", C.c_int * TUNE_MAX),
+                ("corr_steps", C.c_int * TUNE_MAX), ("corr_us", (C.c_double * 4) * TUNE_MAX)]
+
+
 vp = C.c_void_p
 i32 = C.c_int
 u16 = C.c_ushort
@@ -153,6 +164,11 @@ PROTOTYPES = [
     ("fdas_batch_spectrum", i32, [vp, i32, vp]),
     ("fdas_plan_enable_timing", i32, [vp, i32]),
     ("fdas_plan_get_timing", i32, [vp, C.POINTER(FdasTiming)]),
+    ("fdas_autotune", i32, [C.POINTER(FdasParams), C.POINTER(i32), i32, C.POINTER(FdasTuneReport)]),
+    ("fdas_corr_variant_count", i32, [i32]),
+    ("fdas_get_corr_variant", i32, [i32]),
+    ("fdas_set_corr_variant", i32, [i32, i32]),
+    ("fdas_resolve_fftlen", i32, [C.POINTER(FdasParams)]),
     ("fdas_server_create", i32, [C.POINTER(FdasServerConfig), C.POINTER(vp)]),
     ("fdas_server_destroy", None, [vp]),
     ("fdas_server_prewarm", i32, [vp, C.POINTER(FdasParams)]),

@@ -355,6 +355,30 @@ class Server:
     __del__ = AccelSearch.__del__
 
 
+def autotune(params: SearchParams, fftlens=()) -> dict:
+    """Measure correlation launch shapes and fundamental FFT lengths for series of
+    ``params.n`` samples; the winners apply to later plans (fftlen 0)."""
+    arr = (C.c_int * max(len(fftlens), 1))(*fftlens)
+    rep = N.FdasTuneReport()
+    cp = params.to_c()
+    check(lib().fdas_autotune(C.byref(cp), arr, len(fftlens), C.byref(rep)), "fdas_autotune")
+    return {
+        "fftlen": rep.fftlen,
+        "ms_per_trial": {rep.cand_fftlen[i]: (rep.cand_ms_per_trial[i] if rep.cand_ms_per_trial[i] >= 0
+                                              else None) for i in range(rep.ncand)},
+        "corr": {rep.corr_len[i]: {"variant": rep.corr_variant[i], "threads": rep.corr_threads[i],
+                                   "steps_per_wg": rep.corr_steps[i],
+                                   "us_per_variant": [u for u in rep.corr_us[i]
+                                                      [:lib().fdas_corr_variant_count(rep.corr_len[i])]]}
+                 for i in range(rep.ncorr)},
+    }
+
+
+def resolve_fftlen(params: SearchParams) -> int:
+    cp = params.to_c()
+    return lib().fdas_resolve_fftlen(C.byref(cp))
+
+
 def stage_powcut(sigma: float, numharm: int, numindep: float = 1.0) -> float:
     """Summed power whose false-alarm probability over ``numindep`` trials equals
     the Gaussian tail beyond ``sigma``.  Each normalised power is Exp(1) under

@@ -164,7 +164,12 @@ struct CorrArgs {
     int out_half;
     int prec;
 };
-int launch_corr(int L, const CorrArgs& a, hipStream_t s);
+// variant < 0: the shape currently selected for length L (autotuner table).
+int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant = -1);
+int corr_variant_count(int L);
+int corr_variant(int L);
+int set_corr_variant(int L, int v);
+void corr_variant_shape(int L, int v, int* threads, int* steps);
 // Reduced-precision copy of a bank of Z rows of length L.
 int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
                      hipStream_t s);
@@ -179,6 +184,9 @@ int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int
 // al (nullptr: hipMalloc).  Implemented in plan.hip.
 int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
                    hipStream_t stream, DevAlloc* al, fdas_plan** out);
+
+// Fundamental FFT length for params (p.fftlen if set; tuned or default else).
+int resolve_fftlen(const fdas_params& p);
 
 // Stage-API kernels (unfused, reference data layout).
 int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,

@@ -56,9 +56,14 @@ static std::vector<cplx> host_twiddles()
 static std::vector<cplx> host_twiddles_for(int n, int t)
 {
     if (t != fft_threads(n)) {
-        if (n == 8192 && t == 512) return host_twiddles<8192, 512>();
-        if (n == 4096 && t == 512) return host_twiddles<4096, 512>();
+        // the other shapes the correlation and segment kernels instantiate
         if (n == 128 && t == 64) return host_twiddles<128, 64>();
+        if (n == 1024 && t == 128) return host_twiddles<1024, 128>();
+        if (n == 2048 && t == 256) return host_twiddles<2048, 256>();
+        if (n == 4096 && t == 512) return host_twiddles<4096, 512>();
+        if (n == 8192 && t == 512) return host_twiddles<8192, 512>();
+        if (n == 8192 && t == 1024) return host_twiddles<8192, 1024>();
+        if (n == 16384 && t == 1024) return host_twiddles<16384, 1024>();
         return {};
     }
     switch (n) {

@@ -226,6 +226,7 @@ int validate(const fdas_params& p)
 // checks that the kernels' index arithmetic stays inside the planes.
 int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
 {
+    if (pl->p.fftlen == 0) pl->p.fftlen = resolve_fftlen(pl->p);
     const fdas_params& p = pl->p;
     FDAS_TRY(validate(p));
     pl->prec = p.precision;

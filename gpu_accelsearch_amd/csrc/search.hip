@@ -16,6 +16,7 @@
 // Stage path (reference data layout, source/bpd_search.cu:228-360): convolve,
 // powers and add_powers kernels plus the normalise-and-spread kernel used by the
 // reference-named entry points.
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 
@@ -315,21 +316,95 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
     }
 }
 
-// Threads of the fused correlation at each length: 512 (16 values per thread) at
-// 4096 and 8192 measured faster than 256 (more waves per SIMD outweigh the extra
-// LDS pass).
-template <int L>
-struct CorrT {
-    static constexpr int T = (L == 4096 || L == 8192) ? 512 : FftCfgS<L>::T;
+// Launch shapes of the fused correlation per FFT length: (threads, steps per
+// workgroup).  Variant 0 is the default; the autotuner may select another per
+// length.  More threads give more waves per SIMD and fewer values per thread
+// (a smaller radix, so one more LDS pass); fewer steps per workgroup shorten the
+// time template registers stay live.  At 16384 the registers are needed for the
+// transform itself, so every step reloads its template row.
+namespace {
+struct CorrShape {
+    int T, G;
 };
+constexpr CorrShape kShapes256[] = {{64, 8}};
+constexpr CorrShape kShapes512[] = {{64, 8}};
+constexpr CorrShape kShapes1024[] = {{64, 8}, {128, 8}};
+constexpr CorrShape kShapes2048[] = {{128, 8}, {256, 8}};
+constexpr CorrShape kShapes4096[] = {{512, 8}, {256, 8}, {512, 4}};
+constexpr CorrShape kShapes8192[] = {{512, 8}, {512, 4}, {1024, 8}};
+constexpr CorrShape kShapes16384[] = {{512, 1}, {1024, 1}};
 
-template <int L, int P, typename OutE>
+int shape_slot(int L)
+{
+    switch (L) {
+        case 256: return 0;
+        case 512: return 1;
+        case 1024: return 2;
+        case 2048: return 3;
+        case 4096: return 4;
+        case 8192: return 5;
+        case 16384: return 6;
+        default: return -1;
+    }
+}
+
+const CorrShape* shapes_of(int L, int* count)
+{
+#define SHAPES(NN, ARR)                                 \
+    case NN:                                            \
+        *count = (int)(sizeof(ARR) / sizeof(ARR[0]));   \
+        return ARR;
+    switch (L) {
+        SHAPES(256, kShapes256)
+        SHAPES(512, kShapes512)
+        SHAPES(1024, kShapes1024)
+        SHAPES(2048, kShapes2048)
+        SHAPES(4096, kShapes4096)
+        SHAPES(8192, kShapes8192)
+        SHAPES(16384, kShapes16384)
+        default: *count = 0; return nullptr;
+    }
+#undef SHAPES
+}
+
+std::atomic<int> g_shape[7];
+}  // namespace
+
+int corr_variant_count(int L)
+{
+    int c = 0;
+    (void)shapes_of(L, &c);
+    return c;
+}
+
+int corr_variant(int L)
+{
+    const int k = shape_slot(L);
+    return k < 0 ? -1 : g_shape[k].load();
+}
+
+int set_corr_variant(int L, int v)
+{
+    const int k = shape_slot(L);
+    if (k < 0 || v < 0 || v >= corr_variant_count(L)) {
+        set_error("no correlation variant %d at FFT length %d", v, L);
+        return FDAS_ERR_INVALID;
+    }
+    g_shape[k].store(v);
+    return FDAS_OK;
+}
+
+void corr_variant_shape(int L, int v, int* threads, int* steps)
+{
+    int c = 0;
+    const CorrShape* sh = shapes_of(L, &c);
+    *threads = (sh && v >= 0 && v < c) ? sh[v].T : 0;
+    *steps = (sh && v >= 0 && v < c) ? sh[v].G : 0;
+}
+
+template <int L, int T, int G, int P, typename OutE>
 static int corr_t(const CorrArgs& a, hipStream_t s)
 {
-    constexpr int T = CorrT<L>::T;
-    // Templates stay in registers across 8 steps up to length 8192; at 16384 the
-    // registers are needed for the transform itself.
-    constexpr int G = L <= 8192 ? 8 : 1;
     using S = typename CStore<P>::T;
     int err = FDAS_OK;
     const cplx* tw = fft_twiddles_t(L, T, &err);
@@ -342,33 +417,52 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
     return check_launch("corr_sub_kernel");
 }
 
-template <int L>
-static int corr_l(const CorrArgs& a, hipStream_t s)
+template <int L, int T, int G>
+static int corr_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
         case PREC_F16:
-            return a.out_half ? corr_t<L, PREC_F16, unsigned short>(a, s)
-                              : corr_t<L, PREC_F16, float>(a, s);
-        case PREC_BF16: return corr_t<L, PREC_BF16, float>(a, s);
-        default: return corr_t<L, PREC_F32, float>(a, s);
+            return a.out_half ? corr_t<L, T, G, PREC_F16, unsigned short>(a, s)
+                              : corr_t<L, T, G, PREC_F16, float>(a, s);
+        case PREC_BF16: return corr_t<L, T, G, PREC_BF16, float>(a, s);
+        default: return corr_t<L, T, G, PREC_F32, float>(a, s);
     }
 }
 
-int launch_corr(int L, const CorrArgs& a, hipStream_t s)
+template <int L, int V = 0>
+static int corr_l(const CorrArgs& a, hipStream_t s, int v)
+{
+    constexpr int K = L == 256 ? 0 : L == 512 ? 1 : L == 1024 ? 2 : L == 2048 ? 3 : L == 4096 ? 4
+                    : L == 8192 ? 5 : 6;
+    constexpr const CorrShape* SH = K == 0 ? kShapes256 : K == 1 ? kShapes512 : K == 2 ? kShapes1024
+                                  : K == 3 ? kShapes2048 : K == 4 ? kShapes4096 : K == 5 ? kShapes8192
+                                  : kShapes16384;
+    constexpr int NV = K == 0 ? 1 : K == 1 ? 1 : K == 2 ? 2 : K == 3 ? 2 : K == 4 ? 3 : K == 5 ? 3 : 2;
+    if constexpr (V < NV) {
+        if (v == V) return corr_p<L, SH[V].T, SH[V].G>(a, s);
+        return corr_l<L, V + 1>(a, s, v);
+    } else {
+        set_error("correlation variant %d out of range at length %d", v, L);
+        return FDAS_ERR_INVALID;
+    }
+}
+
+int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
 {
     if (a.nsteps <= 0) return FDAS_OK;
     if (a.out_half && a.prec != PREC_F16) {
         set_error("FP16 power planes need FP16 inputs");
         return FDAS_ERR_INVALID;
     }
+    const int v = variant >= 0 ? variant : corr_variant(L);
     switch (L) {
-        case 256: return corr_l<256>(a, s);
-        case 512: return corr_l<512>(a, s);
-        case 1024: return corr_l<1024>(a, s);
-        case 2048: return corr_l<2048>(a, s);
-        case 4096: return corr_l<4096>(a, s);
-        case 8192: return corr_l<8192>(a, s);
-        case 16384: return corr_l<16384>(a, s);
+        case 256: return corr_l<256>(a, s, v);
+        case 512: return corr_l<512>(a, s, v);
+        case 1024: return corr_l<1024>(a, s, v);
+        case 2048: return corr_l<2048>(a, s, v);
+        case 4096: return corr_l<4096>(a, s, v);
+        case 8192: return corr_l<8192>(a, s, v);
+        case 16384: return corr_l<16384>(a, s, v);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;

@@ -154,7 +154,10 @@ typedef struct fdas_params {
     float dt;           /* sample time, s; rlo = floor(n*dt) as in the driver */
     int zmax;           /* 0..1200, even                                      */
     int numharm;        /* 1, 2, 4, 8 or 16                                   */
-    int fftlen;         /* fundamental FFT length, 1024..16384                */
+    int fftlen;         /* fundamental FFT length, 1024..16384; 0: the length
+                           fdas_autotune chose for this (zmax, numharm,
+                           precision), else the smallest valid length whose
+                           template overhang is at most 1/8 of it           */
     int uselen;         /* 0: largest even U that fits fftlen                 */
     int precision;      /* FDAS_PREC_*                                        */
     int chunk_steps;    /* steps per batch of launches, 0: automatic          */
@@ -262,6 +265,37 @@ typedef struct fdas_timing {
 } fdas_timing;
 int fdas_plan_enable_timing(fdas_plan *plan, int enable);
 int fdas_plan_get_timing(fdas_plan *plan, fdas_timing *out);
+
+/* ------------------------------------------------------------------------ */
+/* Autotuning                                                                */
+/* ------------------------------------------------------------------------ */
+/* Measures, on this device and synthetic data: (1) every launch shape of the
+ * fused correlation (threads per workgroup = radix of the LDS passes, steps per
+ * workgroup = template-block tile) at each FFT length the candidates use, and
+ * selects the fastest process-wide; (2) whole DM trials of base->n samples at
+ * each candidate fundamental FFT length (nfftlen 0: 1024..16384), remembering
+ * the fastest for plans with fftlen 0 and the same (zmax, numharm, precision). */
+#define FDAS_TUNE_MAX 8
+typedef struct fdas_tune_report {
+    int fftlen;                              /* chosen fundamental length        */
+    int ncand;
+    int cand_fftlen[FDAS_TUNE_MAX];
+    double cand_ms_per_trial[FDAS_TUNE_MAX]; /* < 0: not valid for the config    */
+    int ncorr;
+    int corr_len[FDAS_TUNE_MAX];             /* FFT lengths whose shape was tuned */
+    int corr_variant[FDAS_TUNE_MAX];         /* selected variant                  */
+    int corr_threads[FDAS_TUNE_MAX];
+    int corr_steps[FDAS_TUNE_MAX];
+    double corr_us[FDAS_TUNE_MAX][4];        /* us per launch per variant (0: n/a) */
+} fdas_tune_report;
+
+int fdas_autotune(const fdas_params *base, const int *fftlens, int nfftlen,
+                  fdas_tune_report *report);
+int fdas_corr_variant_count(int fftlen);
+int fdas_get_corr_variant(int fftlen);
+int fdas_set_corr_variant(int fftlen, int variant);
+/* The fundamental FFT length a plan with these parameters would use. */
+int fdas_resolve_fftlen(const fdas_params *params);
 
 /* ------------------------------------------------------------------------ */
 /* 3. Serving: a continuous-batching request loop                            */

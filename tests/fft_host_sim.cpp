@@ -58,12 +58,16 @@ extern "C" int fft_sim(int n, int t, int dir, const float* in, float* out)
     CASE(256, 64)
     CASE(512, 64)
     CASE(1024, 64)
+    CASE(1024, 128)
     CASE(2048, 128)
+    CASE(2048, 256)
     CASE(4096, 256)
     CASE(4096, 512)
     CASE(8192, 256)
     CASE(8192, 512)
+    CASE(8192, 1024)
     CASE(16384, 512)
+    CASE(16384, 1024)
 #undef CASE
     return -1;
 }

@@ -1,0 +1,70 @@
+"""Autotuner: every correlation launch shape agrees with the oracle (1e-4 of
+max(P_ref, 1)), and the tuner's choices apply to later plans."""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+POWER_TOL = 1e-4
+ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384)
+
+
+@pytest.fixture(scope="module")
+def fd():
+    import gpu_accelsearch_amd as fd
+    fd.lib()
+    yield fd
+    for L in ALL_LENGTHS:
+        fd.lib().fdas_set_corr_variant(L, 0)
+
+
+@pytest.mark.parametrize("n,zmax,numharm,fftlen", [
+    (1 << 20, 50, 2, 2048),     # correlation lengths 2048 and 1024
+    (1 << 20, 200, 2, 8192),    # 8192 and 4096
+    (1 << 21, 200, 1, 16384),   # 16384
+])
+def test_every_corr_variant_matches_oracle(fd, oracle, n, zmax, numharm, fftlen):
+    from gpu_accelsearch_amd import synth
+    x = synth.plain_series(n, seed=21, r0=30000.3, z0=zmax / 2, power=80.0)
+    ref = oracle.Search(x, n, 6.4e-5, zmax, numharm, fftlen)
+    L = fd.lib()
+    try:
+        with fd.AccelSearch(fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen)) as s:
+            lens = sorted({hm[2] for hm in s.harmonics})
+            nv = max(L.fdas_corr_variant_count(l) for l in lens)
+            steps = [0, s.nsteps // 2]
+            want = [ref.step_plane(st) for st in steps]
+            s.prepare(x)
+            for v in range(nv):
+                for l in lens:
+                    assert L.fdas_set_corr_variant(l, min(v, L.fdas_corr_variant_count(l) - 1)) == 0
+                print_case("corr variant", v, "lengths", lens)
+                for st, w in zip(steps, want):
+                    got = s.run_steps(st, 1)[0]
+                    e = float(np.max(np.abs(got - w) / np.maximum(w, 1.0)))
+                    assert e <= POWER_TOL, (v, st, e)
+    finally:
+        for l in ALL_LENGTHS:
+            L.fdas_set_corr_variant(l, 0)
+        ref.close()
+
+
+def test_autotune_selects_and_applies(fd):
+    p = fd.SearchParams(n=1 << 20, zmax=50, numharm=4, fftlen=0)
+    print_case("autotune")
+    rep = fd.autotune(p, fftlens=(1024, 2048, 4096))
+    print(rep)
+    assert rep["fftlen"] in (1024, 2048, 4096)
+    times = rep["ms_per_trial"]
+    assert all(times[k] > 0 for k in (1024, 2048, 4096))
+    assert times[rep["fftlen"]] == min(times.values())
+    assert fd.resolve_fftlen(p) == rep["fftlen"]
+    for l, c in rep["corr"].items():
+        us = c["us_per_variant"]
+        assert fd.lib().fdas_get_corr_variant(l) == c["variant"]
+        if len(us) > 1:
+            assert us[c["variant"]] == min(us)
+    with fd.AccelSearch(p) as s:
+        assert s.info.fftlen[0] == rep["fftlen"]

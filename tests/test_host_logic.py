@@ -65,6 +65,25 @@ def test_batch_rejects_bad_lengths(lengths, code):
     assert fd.last_error()
 
 
+def test_fftlen_defaults_and_corr_variants():
+    """fftlen 0 resolves to the smallest valid length with template overhang
+    4 hw(zmax) + 2 at most 1/8 of it; correlation shape tables are host state."""
+    import gpu_accelsearch_amd as fd
+    L = fd.lib()
+    want = {(50, 4): 2048, (50, 16): 2048, (100, 8): 2048, (200, 8): 4096, (300, 8): 8192,
+            (600, 16): 16384}
+    for (zmax, numharm), fl in want.items():
+        assert fd.resolve_fftlen(fd.SearchParams(n=1 << 20, zmax=zmax, numharm=numharm, fftlen=0)) == fl
+    assert fd.resolve_fftlen(fd.SearchParams(n=1 << 20, zmax=200, numharm=8, fftlen=8192)) == 8192
+    counts = {l: L.fdas_corr_variant_count(l) for l in (256, 512, 1024, 2048, 4096, 8192, 16384)}
+    assert all(c >= 1 for c in counts.values()) and counts[8192] >= 2
+    assert L.fdas_corr_variant_count(3000) == 0
+    assert L.fdas_get_corr_variant(8192) == 0
+    assert L.fdas_set_corr_variant(8192, counts[8192]) == 1
+    assert L.fdas_set_corr_variant(8192, 1) == 0 and L.fdas_get_corr_variant(8192) == 1
+    assert L.fdas_set_corr_variant(8192, 0) == 0
+
+
 def test_uselen_setting():
     import gpu_accelsearch_amd as fd
     L = fd.lib()
@@ -142,8 +161,9 @@ def fft_sim(tmp_path_factory):
 
 
 @pytest.mark.parametrize("n,t", [(64, 16), (128, 32), (128, 64), (256, 64), (512, 64),
-                                 (1024, 64), (2048, 128), (4096, 256), (4096, 512), (8192, 256),
-                                 (8192, 512), (16384, 512)])
+                                 (1024, 64), (1024, 128), (2048, 128), (2048, 256), (4096, 256),
+                                 (4096, 512), (8192, 256), (8192, 512), (8192, 1024),
+                                 (16384, 512), (16384, 1024)])
 def test_workgroup_fft_index_arithmetic(fft_sim, n, t):
     """Every (length, threads) shape the library instantiates."""
     rng = np.random.default_rng(n)
