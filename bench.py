@@ -68,16 +68,116 @@ def cpu_baseline(series, n, zmax, numharm, fftlen, steps):
                       f"the trial ({t_steps:.1f} s); cells/s over the sampled steps"}
 
 
+def stage_form(spec, n, zmax, numharm, fftlen, info, steps):
+    """The reference's per-step stage sequence (AccelSearch.cu:217-241) on the GPU:
+    host segment fetch, unfused forward FFTs, complex correlation planes written
+    to memory, batched inverse FFTs, powers and harmonic sums, one step at a time."""
+    import gpu_accelsearch_amd as fd
+    fd.accel.fdas_set_uselen(info.uselen)
+    try:
+        bank = fd.createSubharmonicInfo(int(np.log2(numharm)) + 1, zmax)
+        pipe = fd.StagePipeline(bank)
+        sel = np.linspace(0, info.nsteps - 1, steps).astype(int)
+        pipe.step(spec, info.rlo, -zmax, info.uselen, info.nbins)   # warm-up
+        t0 = time.perf_counter()
+        for st in sel:
+            pipe.step(spec, info.rlo + int(st) * (info.uselen // 2), -zmax, info.uselen, info.nbins)
+        dt = (time.perf_counter() - t0) / len(sel)
+        pipe.close()
+        bank.free()
+    finally:
+        fd.accel.fdas_set_uselen(32000)
+    return {"ms_per_step": dt * 1e3, "ms_per_trial": dt * 1e3 * info.nsteps,
+            "dm_trials_per_s": 1.0 / (dt * info.nsteps), "sampled_steps": int(len(sel)),
+            "note": "per-trial time = sampled mean step time x steps of the trial"}
+
+
+def batch_form(series, n, zmax, numharm, trials, precision, fftlen, reps=3):
+    """Many DM trials in one plan, replayed from a captured hipGraph."""
+    import gpu_accelsearch_amd as fd
+    P = fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen, precision=precision,
+                        powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+    with fd.BatchSearch(P, [n] * trials, graph=True) as b:
+        for t in range(trials):
+            b.upload(t, series)
+        b.run()                       # eager run + capture
+        b.run()                       # first replay
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        return {"trials": trials, "precision": precision, "fftlen": fftlen, "graph": True,
+                "dm_trials_per_s": trials * reps / dt, "ms_per_trial": dt * 1e3 / (trials * reps),
+                "cells_per_s": b.info.cells * reps / dt, "device_bytes": int(b.info.device_bytes)}
+
+
+def serving_form(count, rate, seed=6):
+    """A bounded Poisson request stream through the continuous-batching server."""
+    import threading
+    import gpu_accelsearch_amd as fd
+    from gpu_accelsearch_amd import synth
+    reqs = synth.serving_stream(count, rate, seed=seed)
+    pool = synth.noise_pool(1 << 25)
+    series = []
+    for r in reqs:
+        x = pool[r["offset"]:r["offset"] + r["n"]].copy()
+        if r["pulsar"]:
+            x += synth.pulse_train(r["n"], 0.37 * r["n"] / 2, 0.5 * r["zmax"], 0.05, 30.0).astype(np.float32)
+        series.append(x)
+    params = [fd.SearchParams(n=r["n"], zmax=r["zmax"], numharm=r["numharm"], fftlen=0,
+                              precision=r["precision"], powcut=(40.0, 45.0, 55.0, 70.0, 90.0))
+              for r in reqs]
+    with fd.Server(max_active=4, chunk_steps=128) as srv:
+        for z in (50, 200, 600):
+            for h in (4, 8, 16):
+                for prec in ("fp32", "fp16", "bf16"):
+                    srv.prewarm(fd.SearchParams(n=1 << 20, zmax=z, numharm=h, fftlen=0, precision=prec))
+        ids = [None] * count
+        t0 = time.perf_counter()
+
+        def feed():
+            for i, (r, p, x) in enumerate(zip(reqs, params, series)):
+                delay = r["t"] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit(p, x)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res = [srv.wait(i)[0] for i in ids]
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    cells = sum(r["cells"] for r in res)
+    return {"requests": count, "offered_rate_per_s": rate, "wall_s": wall,
+            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+            "dm_trials_per_s": st["trials_per_s"], "cells_per_s": cells / wall,
+            "max_active_seen": st["max_active_seen"], "memory_peak_bytes": st["memory_peak"],
+            "memory_budget_bytes": st["memory_bytes"], "failed": st["failed"],
+            "mix": "n log-uniform [2^20, 2^25], zmax {50,200,600}, numharm {4,8,16}, "
+                   "precision {fp32,fp16,bf16}, 5% with a pulsar; fftlen per request from the "
+                   "autotuned/default table"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed DM trials")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=12, help="oracle steps sampled")
+    ap.add_argument("--cpu-steps", type=int, default=96, help="oracle steps sampled")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
                     help="storage of spectra segments and templates (arithmetic is FP32)")
+    ap.add_argument("--batch-trials", type=int, default=8,
+                    help="DM trials of the batched reduced-precision form (0: skip)")
+    ap.add_argument("--stage-steps", type=int, default=12,
+                    help="steps sampled for the reference-structured stage form (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=24,
+                    help="requests of the Poisson serving stream (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling profiler byte counters)")
@@ -87,6 +187,7 @@ def main() -> int:
               "DM trials shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
         return 2
 
+    import gpu_accelsearch_amd as fd
     from gpu_accelsearch_amd import AccelSearch, DeviceArray, SearchParams, synchronize
     from gpu_accelsearch_amd import synth
 
@@ -95,6 +196,8 @@ def main() -> int:
     params = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen,
                           chunk_steps=args.chunk, precision=args.precision,
                           powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+    # correlation launch shapes measured for this configuration's FFT lengths
+    tune = fd.autotune(params, fftlens=(fftlen,))
     s = AccelSearch(params)
     d_series = DeviceArray.from_host(series)
     for _ in range(args.warmup):
@@ -159,7 +262,26 @@ def main() -> int:
                            "frac": ab["trial"] / per_trial / HBM_PEAK},
         "kernels": kern,
         "candidates": int(len(cands)),
+        "autotune": {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"]}
+                     for k, v in tune["corr"].items()},
     }
+    spec = s.spectrum() if args.stage_steps > 0 else None
+    s.close()
+    del d_series
+    if args.stage_steps > 0:
+        out["stage_form_fp32"] = stage_form(spec, n, zmax, numharm, fftlen, info, args.stage_steps)
+    if args.batch_trials > 0:
+        rp = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=0, precision="bf16")
+        bt = fd.autotune(rp, fftlens=(4096, 8192, 16384))
+        out["batch_bf16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "bf16", bt["fftlen"])
+        out["batch_bf16"]["fftlen_ms_per_trial"] = bt["ms_per_trial"]
+        out["batch_fp32"] = batch_form(series, n, zmax, numharm, args.batch_trials, "fp32", fftlen)
+        ref = out.get("stage_form_fp32", {}).get("dm_trials_per_s")
+        out["batch_bf16"]["vs_fused_fp32_single"] = out["batch_bf16"]["dm_trials_per_s"] * per_trial
+        if ref:
+            out["batch_bf16"]["vs_stage_form_fp32"] = out["batch_bf16"]["dm_trials_per_s"] / ref
+    if args.serve_requests > 0:
+        out["serving"] = serving_form(args.serve_requests, args.serve_rate)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(series, n, zmax, numharm, fftlen, args.cpu_steps)
         out["cpu_baseline"]["vs_gpu"] = cells_per_s / out["cpu_baseline"]["value"]
@@ -170,7 +292,6 @@ def main() -> int:
         lib().fdas_memcpy_dtod(b.ptr, a.ptr, nb)
         synchronize()
         out["copy_probe_bytes"] = nb
-    s.close()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -78,3 +78,33 @@ def production_series(n: int = 1 << 23, seed: int = 2, pulsars=PRODUCTION_PULSAR
 def steps_covering(r: float, rlo: int, uselen: int) -> int:
     """Index of the step whose fundamental columns contain Fourier frequency r."""
     return int((r - rlo) // (uselen // 2))
+
+
+def log_uniform_lengths(count: int, lo: int, hi: int, seed: int = 3) -> list:
+    """Even series lengths drawn log-uniformly in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(np.log2(lo), np.log2(hi), count)
+    return [int(round(2.0 ** v / 2.0)) * 2 for v in u]
+
+
+def noise_pool(n: int, seed: int = 5, dt: float = DT) -> np.ndarray:
+    """One long N(0,1) + red-noise series; requests take windows of it."""
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(n) + red_noise(n, rng, dt)).astype(np.float32)
+
+
+def serving_stream(count: int, rate: float, max_len: int = 1 << 25, min_len: int = 1 << 20,
+                   seed: int = 6, pulsar_frac: float = 0.05):
+    """Request descriptions of a Poisson stream: arrival time (s), length, zmax,
+    numharm, storage precision and whether a pulsar is injected."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    lens = log_uniform_lengths(count, min_len, max_len, seed + 1)
+    out = []
+    for i in range(count):
+        out.append({"t": float(t[i]), "n": lens[i], "zmax": int(rng.choice([50, 200, 600])),
+                    "numharm": int(rng.choice([4, 8, 16])),
+                    "precision": str(rng.choice(["fp32", "fp16", "bf16"])),
+                    "pulsar": bool(rng.uniform() < pulsar_frac),
+                    "offset": int(rng.integers(0, max_len - lens[i] + 1))})
+    return out
