@@ -49,6 +49,30 @@ def alg_bytes(info, n: int) -> dict:
     }
 
 
+def corr_flops(info) -> dict:
+    """FFT-model FLOPs per DM trial of the fused correlation kernels: per
+    (template, step) 5 L log2 L for the inverse transform + 6 L for the product,
+    + 3 per kept power."""
+    S, H = info.nsteps, info.numharm
+    sub = sum(info.sub_cells[h] for h in range(1, H))
+    f = lambda h: info.numzs[h] * (5 * info.fftlen[h] * np.log2(info.fftlen[h]) + 6 * info.fftlen[h])
+    return {"corr_sub": S * sum(f(h) for h in range(1, H)) + 3 * sub,
+            "corr_fund": S * f(0) + 3 * info.cells}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel category from the committed counter
+    summary (two rocprofv3 --pmc runs of this script, FETCH_SIZE and WRITE_SIZE,
+    scaled by the same counters on a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "production_trial_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d["per_launch_bytes"][kernel]["total"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(series, n, zmax, numharm, fftlen, steps):
     """The CPU oracle (FP64 restatement, one core) on a bounded sample."""
     from oracle import oracle as O
@@ -178,6 +202,8 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the Poisson serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default correlation shapes (no autotuner launches)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB "
                          "(a kernel of known traffic for scaling profiler byte counters)")
@@ -197,7 +223,7 @@ def main() -> int:
                           chunk_steps=args.chunk, precision=args.precision,
                           powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
     # correlation launch shapes measured for this configuration's FFT lengths
-    tune = fd.autotune(params, fftlens=(fftlen,))
+    tune = None if args.no_tune else fd.autotune(params, fftlens=(fftlen,))
     s = AccelSearch(params)
     d_series = DeviceArray.from_host(series)
     for _ in range(args.warmup):
@@ -256,14 +282,18 @@ def main() -> int:
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(dom),
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
+        "valu": {k: {"tflops": fl / (tm[k][1] / prof_trials * 1e-3) / 1e12,
+                     "frac_of_157.3": fl / (tm[k][1] / prof_trials * 1e-3) / 157.3e12,
+                     "flops_per_trial": fl}
+                 for k, fl in corr_flops(info).items()},
         "trial_roofline": {"alg_bytes": ab["trial"],
                            "frac": ab["trial"] / per_trial / HBM_PEAK},
         "kernels": kern,
         "candidates": int(len(cands)),
-        "autotune": {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"]}
-                     for k, v in tune["corr"].items()},
+        "autotune": None if tune is None else
+        {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"]} for k, v in tune["corr"].items()},
     }
     spec = s.spectrum() if args.stage_steps > 0 else None
     s.close()
