@@ -137,6 +137,33 @@ def batch_form(series, n, zmax, numharm, trials, precision, fftlen, reps=3):
                 "cells_per_s": b.info.cells * reps / dt, "device_bytes": int(b.info.device_bytes)}
 
 
+def batch_workload_form(trials, precision, reps=1):
+    """The batching workload: uneven lengths log-uniform in [2^21, 2^23], zmax 200,
+    16 harmonics, autotuned fftlen, one plan replayed from a captured hipGraph."""
+    import gpu_accelsearch_amd as fd
+    from gpu_accelsearch_amd import synth
+    lengths, xs = synth.batch_workload(trials)
+    P = fd.SearchParams(n=lengths[0], zmax=200, numharm=16, fftlen=0, precision=precision,
+                        powcut=(35.0, 40.0, 50.0, 65.0, 90.0))
+    rep = fd.autotune(P, fftlens=(4096, 8192, 16384))
+    with fd.BatchSearch(P, lengths, graph=True) as b:
+        for t, x in enumerate(xs):
+            b.upload(t, x)
+        b.run()                       # eager run + capture
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            b.run()
+        b.synchronize()
+        dt = time.perf_counter() - t0
+        ncand = len(b.candidates())
+        return {"trials": trials, "precision": precision, "fftlen": rep["fftlen"], "graph": True,
+                "lengths": "log-uniform [2^21, 2^23], 10% with an injected accelerated sinusoid",
+                "numharm": 16, "zmax": 200, "dm_trials_per_s": trials * reps / dt,
+                "cells_per_s": b.info.cells * reps / dt, "ms_per_run": dt * 1e3 / reps,
+                "candidates": ncand, "fftlen_ms_per_trial_at_first_length": rep["ms_per_trial"]}
+
+
 def serving_form(count, rate, seed=6):
     """A bounded Poisson request stream through the continuous-batching server."""
     import threading
@@ -199,6 +226,8 @@ def main() -> int:
                     help="DM trials of the batched reduced-precision form (0: skip)")
     ap.add_argument("--stage-steps", type=int, default=12,
                     help="steps sampled for the reference-structured stage form (0: skip)")
+    ap.add_argument("--batch-workload-trials", type=int, default=256,
+                    help="trials of the uneven-length 16-harmonic batch workload (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=24,
                     help="requests of the Poisson serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -310,6 +339,8 @@ def main() -> int:
         out["batch_bf16"]["vs_fused_fp32_single"] = out["batch_bf16"]["dm_trials_per_s"] * per_trial
         if ref:
             out["batch_bf16"]["vs_stage_form_fp32"] = out["batch_bf16"]["dm_trials_per_s"] / ref
+    if args.batch_workload_trials > 0:
+        out["batch_workload_bf16"] = batch_workload_form(args.batch_workload_trials, "bf16")
     if args.serve_requests > 0:
         out["serving"] = serving_form(args.serve_requests, args.serve_rate)
     if not args.no_cpu:

@@ -108,3 +108,26 @@ def serving_stream(count: int, rate: float, max_len: int = 1 << 25, min_len: int
                     "pulsar": bool(rng.uniform() < pulsar_frac),
                     "offset": int(rng.integers(0, max_len - lens[i] + 1))})
     return out
+
+
+def add_chirp(x: np.ndarray, r0: float, z0: float, power: float) -> np.ndarray:
+    """x plus one accelerated sinusoid of normalised Fourier power ``power``."""
+    n = len(x)
+    amp = np.sqrt(4.0 * power / n)
+    return (x + amp * np.cos(2.0 * np.pi * chirp_phase(n, r0, z0))).astype(np.float32)
+
+
+def batch_workload(count: int, lo: int = 1 << 21, hi: int = 1 << 23, seed: int = 4,
+                   pulsar_frac: float = 0.1):
+    """Series of a DM-trial batch: lengths log-uniform in [lo, hi], windows of one
+    N(0,1) + red-noise series, a fraction carrying an accelerated sinusoid."""
+    rng = np.random.default_rng(seed)
+    lengths = log_uniform_lengths(count, lo, hi, seed)
+    pool = noise_pool(hi, seed=seed + 1)
+    xs = []
+    for n in lengths:
+        x = pool[:n]
+        if rng.uniform() < pulsar_frac:
+            x = add_chirp(x, rng.uniform(2e4, 0.4 * n / 2), rng.uniform(-150, 150), 60.0)
+        xs.append(x)
+    return lengths, xs

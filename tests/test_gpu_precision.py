@@ -81,3 +81,59 @@ def test_precision_production_candidates(fd, oracle, mode):
         print(f"{mode}: {len(want)} oracle candidates, {len(got)} found, "
               f"{len(ambiguous)} within the error of the threshold")
         ref.close()
+
+
+# Precision workload: 2^24 samples of N(0,1) + red noise + 5 accelerated pulsars
+# whose first harmonics span ~6 to ~30 sigma; zmax 300, 8 harmonics, fftlen 8192.
+C4_PULSARS = ((120000.3, 30.0, 21.0), (250000.6, -25.0, 45.0), (410000.2, 12.0, 90.0),
+              (600000.8, -35.0, 200.0), (900000.4, 20.0, 455.0))
+C4_POWCUT = (30.0, 36.0, 45.0, 60.0, 0.0)
+C4_TOL = {"fp32": 1e-4, "fp16": 2e-3, "bf16": 2e-2}
+
+
+@pytest.fixture(scope="module")
+def c4(oracle):
+    from gpu_accelsearch_amd import synth
+    n = 1 << 24
+    x = synth.production_series(n, seed=8, pulsars=C4_PULSARS)
+    ref = oracle.Search(x, n, 6.4e-5, 300, 8, 8192)
+    steps = sorted({synth.steps_covering(8 * r0, ref.rlo, ref.uselen) for r0, _, _ in C4_PULSARS})
+    planes = {st: ref.step_plane(st, stages=True) for st in steps}
+    ref.close()
+    return x, n, steps, planes
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp16", "bf16"])
+def test_precision_workload(fd, c4, mode):
+    """Max relative power error and candidate agreement per storage mode."""
+    x, n, steps, planes = c4
+    print_case("precision workload 2^24 zmax 300", mode)
+    P = fd.SearchParams(n=n, zmax=300, numharm=8, fftlen=8192, precision=mode, powcut=C4_POWCUT)
+    with fd.AccelSearch(P) as s:
+        assert s.uselen == 7470
+        s.prepare(x)
+        worst = worst_big = 0.0
+        for st in steps:
+            got = s.run_steps(st, 1)[0].astype(np.float64)
+            want = planes[st][0]
+            worst = max(worst, power_err(got, want))
+            big = want >= 1.0
+            worst_big = max(worst_big, float(np.max(np.abs(got[big] - want[big]) / want[big])))
+        s.search(x)
+        got = {(int(c["step"]), int(c["z_index"]), int(c["r_index"]), int(c["stage"]))
+               for c in s.candidates() if int(c["step"]) in steps}
+    want, ambiguous = set(), set()
+    for st in steps:
+        stages = planes[st][1]
+        for sg in range(4):
+            z, i = np.nonzero(stages[sg] > C4_POWCUT[sg])
+            want |= {(st, int(a), int(b), sg) for a, b in zip(z, i)}
+            z, i = np.nonzero(np.abs(stages[sg] - C4_POWCUT[sg]) <=
+                              C4_TOL[mode] * np.maximum(stages[sg], 1.0))
+            ambiguous |= {(st, int(a), int(b), sg) for a, b in zip(z, i)}
+    agree = got - ambiguous == want - ambiguous
+    print(f"PRECISION {mode}: max |dP|/max(P,1) {worst:.3e}, max |dP|/P (P>=1) {worst_big:.3e}, "
+          f"oracle candidates {len(want)}, found {len(got)}, within error of threshold "
+          f"{len(ambiguous)}, agree {agree}")
+    assert worst <= C4_TOL[mode], worst
+    assert len(want) > 0 and agree
