@@ -5,7 +5,8 @@ ABI in include/fdas.h.  Importing the package does not load the library; the
 first call that needs it does, and raises if it has not been built.
 """
 from ._native import FdasError, last_error, lib  # noqa: F401
-from .accel import (AccelSearch, BatchSearch, autotune, resolve_fftlen, CANDIDATE_DTYPE, DeviceArray, SearchParams,  # noqa: F401
+from .accel import (ACCELCAND_DTYPE, AccelSearch, BatchSearch, autotune, candidate_sigma,
+                    numindep, power_for_sigma, resolve_fftlen, sift_raw, CANDIDATE_DTYPE, DeviceArray, SearchParams,  # noqa: F401
                     Server, StagePipeline, SubharmonicBank, calc_required_r, calc_required_z,
                     calculateFFTLength, complexToComplexFFT, createSubharmonicInfo, deredden,
                     generate_harmonics_ffdot, getTotalHarmonics, index_from_r, index_from_z,
@@ -19,4 +20,5 @@ __all__ = [
     "calc_required_r", "index_from_r", "index_from_z", "getTotalHarmonics",
     "z_response_halfwidth", "calculateFFTLength", "stage_powcut", "synchronize", "lib",
     "FdasError", "last_error", "CANDIDATE_DTYPE", "autotune", "resolve_fftlen",
+    "ACCELCAND_DTYPE", "candidate_sigma", "power_for_sigma", "numindep", "sift_raw",
 ]

@@ -93,6 +93,24 @@ class FdasServerStats(C.Structure):
                 ("busy_s", C.c_double), ("trials_per_s", C.c_double)]
 
 
+class FdasAccelcand(C.Structure):
+    _fields_ = [("trial", C.c_int), ("numharm", C.c_int), ("step", C.c_int), ("z_index", C.c_int),
+                ("r_index", C.c_int), ("stage", C.c_int), ("power", C.c_float), ("sigma", C.c_float),
+                ("r", C.c_double), ("z", C.c_double), ("opt_r", C.c_double), ("opt_z", C.c_double),
+                ("opt_power", C.c_float), ("opt_sigma", C.c_float),
+                ("harm_power", C.c_float * MAX_HARM), ("freq", C.c_double), ("fdot", C.c_double),
+                ("period", C.c_double), ("pdot", C.c_double)]
+
+
+class FdasSiftParams(C.Structure):
+    _fields_ = [("sigma_min", C.c_double), ("r_tol", C.c_double), ("z_tol", C.c_double),
+                ("max_ratio", C.c_int)]
+
+
+class FdasSiftTrial(C.Structure):
+    _fields_ = [("n", C.c_long), ("nbins", C.c_long), ("rlo", C.c_int), ("dt", C.c_float)]
+
+
 TUNE_MAX = 8
 
 
@@ -164,6 +182,15 @@ PROTOTYPES = [
     ("fdas_batch_spectrum", i32, [vp, i32, vp]),
     ("fdas_plan_enable_timing", i32, [vp, i32]),
     ("fdas_plan_get_timing", i32, [vp, C.POINTER(FdasTiming)]),
+    ("fdas_candidate_sigma", C.c_double, [C.c_double, i32, C.c_double]),
+    ("fdas_power_for_sigma", C.c_double, [C.c_double, i32, C.c_double]),
+    ("fdas_numindep", C.c_double, [C.c_double, C.c_double, i32, i32]),
+    ("fdas_stage_powcuts", i32, [vp, i32, C.c_double, C.POINTER(C.c_float)]),
+    ("fdas_sift", i32, [vp, vp, i32, C.POINTER(FdasSiftParams), vp, i32, C.POINTER(i32)]),
+    ("fdas_sift_raw", i32, [C.POINTER(FdasSiftTrial), i32, i32, i32, i32, vp, i32,
+                            C.POINTER(FdasSiftParams), vp, i32, C.POINTER(i32)]),
+    ("fdas_optimize", i32, [vp, vp, i32]),
+    ("fdas_rz_power", i32, [vp, i32, vp, vp, i32, vp]),
     ("fdas_autotune", i32, [C.POINTER(FdasParams), C.POINTER(i32), i32, C.POINTER(FdasTuneReport)]),
     ("fdas_corr_variant_count", i32, [i32]),
     ("fdas_get_corr_variant", i32, [i32]),

@@ -98,7 +98,88 @@ CANDIDATE_DTYPE = np.dtype([("step", np.int32), ("z_index", np.int32), ("r_index
                             ("stage", np.int32), ("power", np.float32), ("trial", np.int32)])
 
 
-class AccelSearch:
+ACCELCAND_DTYPE = np.dtype([
+    ("trial", np.int32), ("numharm", np.int32), ("step", np.int32), ("z_index", np.int32),
+    ("r_index", np.int32), ("stage", np.int32), ("power", np.float32), ("sigma", np.float32),
+    ("r", np.float64), ("z", np.float64), ("opt_r", np.float64), ("opt_z", np.float64),
+    ("opt_power", np.float32), ("opt_sigma", np.float32), ("harm_power", np.float32, (N.MAX_HARM,)),
+    ("freq", np.float64), ("fdot", np.float64), ("period", np.float64), ("pdot", np.float64)],
+    align=True)
+assert ACCELCAND_DTYPE.itemsize == C.sizeof(N.FdasAccelcand)
+
+
+def candidate_sigma(power: float, numharm: int, numindep: float) -> float:
+    """Equivalent Gaussian sigma of a summed power of ``numharm`` harmonics over
+    ``numindep`` independent trials."""
+    return lib().fdas_candidate_sigma(float(power), int(numharm), float(numindep))
+
+
+def power_for_sigma(sigma: float, numharm: int, numindep: float) -> float:
+    return lib().fdas_power_for_sigma(float(sigma), int(numharm), float(numindep))
+
+
+def numindep(rlo: float, rhi: float, numz: int, stage: int) -> float:
+    return lib().fdas_numindep(float(rlo), float(rhi), int(numz), int(stage))
+
+
+def _sift_params(sigma_min, r_tol, z_tol, max_ratio):
+    return N.FdasSiftParams(float(sigma_min), float(r_tol), float(z_tol), int(max_ratio))
+
+
+def _run_sift(call, cands: np.ndarray) -> np.ndarray:
+    cands = np.ascontiguousarray(cands, dtype=CANDIDATE_DTYPE)
+    n_out = C.c_int()
+    check(call(cands.ctypes.data, len(cands), None, 0, C.byref(n_out)), "fdas_sift")
+    out = np.zeros(n_out.value, ACCELCAND_DTYPE)
+    if n_out.value:
+        check(call(cands.ctypes.data, len(cands), out.ctypes.data, n_out.value, C.byref(n_out)),
+              "fdas_sift")
+    return out
+
+
+def sift_raw(trials, zmax: int, uselen: int, numz: int, cands: np.ndarray, sigma_min: float = 0.0,
+             r_tol: float = 0.0, z_tol: float = 0.0, max_ratio: int = 0) -> np.ndarray:
+    """Sift search candidates on explicit trial geometry: ``trials`` is a list of
+    (n, nbins, rlo, dt)."""
+    tr = (N.FdasSiftTrial * len(trials))(*[N.FdasSiftTrial(int(a), int(b), int(c), float(d))
+                                            for a, b, c, d in trials])
+    prm = _sift_params(sigma_min, r_tol, z_tol, max_ratio)
+    return _run_sift(lambda ci, ni, co, cap, no: lib().fdas_sift_raw(
+        tr, len(trials), zmax, uselen, numz, ci, ni, C.byref(prm), co, cap, no), cands)
+
+
+class _CandidateOps:
+    """Significance, sifting and optimisation on a plan's candidates."""
+
+    def stage_powcuts(self, sigma: float, trial: int = 0):
+        pc = (C.c_float * N.MAX_STAGES)()
+        check(lib().fdas_stage_powcuts(self.handle, trial, float(sigma), pc), "fdas_stage_powcuts")
+        return tuple(float(v) for v in pc)
+
+    def sift(self, cands: np.ndarray | None = None, sigma_min: float = 0.0, r_tol: float = 0.0,
+             z_tol: float = 0.0, max_ratio: int = 0) -> np.ndarray:
+        if cands is None:
+            cands = self.candidates()
+        prm = _sift_params(sigma_min, r_tol, z_tol, max_ratio)
+        return _run_sift(lambda ci, ni, co, cap, no: lib().fdas_sift(
+            self.handle, ci, ni, C.byref(prm), co, cap, no), cands)
+
+    def optimize(self, acc: np.ndarray) -> np.ndarray:
+        """Optimised copy of sifted candidates (needs the spectra: run after a search)."""
+        acc = np.ascontiguousarray(acc, dtype=ACCELCAND_DTYPE).copy()
+        check(lib().fdas_optimize(self.handle, acc.ctypes.data, len(acc)), "fdas_optimize")
+        return acc
+
+    def rz_power(self, r, z, trial: int = 0) -> np.ndarray:
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.ascontiguousarray(z, dtype=np.float64)
+        out = np.zeros(len(r), np.float32)
+        check(lib().fdas_rz_power(self.handle, trial, r.ctypes.data, z.ctypes.data, len(r),
+                                  out.ctypes.data), "fdas_rz_power")
+        return out
+
+
+class AccelSearch(_CandidateOps):
     """Fused f-fdot search of one series shape (templates stay on the device)."""
 
     def __init__(self, params: SearchParams):
@@ -197,7 +278,7 @@ class AccelSearch:
             pass
 
 
-class BatchSearch:
+class BatchSearch(_CandidateOps):
     """Many DM trials (any even lengths, one zmax/numharm/fftlen) in one plan.
 
     The reference runs AccelSearch.cu's loop once per series; here every

@@ -188,6 +188,19 @@ int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, 
 // Fundamental FFT length for params (p.fftlen if set; tuned or default else).
 int resolve_fftlen(const fdas_params& p);
 
+// Candidates (sift.hip).
+double candidate_sigma(double power, int numharm, double numindep);
+double power_for_sigma(double sigma, int numharm, double numindep);
+double numindep_count(double rlo, double rhi, int numz, int stage);
+int sift_candidates(const fdas_sift_trial* trials, int ntrials, int zmax, int uselen, int numz,
+                    const fdas_candidate* in, int n_in, const fdas_sift_params* prm,
+                    fdas_accelcand* out, int cap, int* n_out);
+void cand_properties(fdas_accelcand* c, double T);
+int optimize_candidates(const cplx* d_spec, const std::vector<long long>& spec_off,
+                        const std::vector<int>& nbins, fdas_accelcand* c, int n, hipStream_t s);
+int rz_powers(const cplx* d_spec, int nbins, const double* r, const double* z, int count,
+              float* out, hipStream_t s);
+
 // Stage-API kernels (unfused, reference data layout).
 int launch_convolve(const cplx* d_fft, const cplx* d_bank, int Z, int L, cplx* d_out,
                     hipStream_t s);

@@ -921,3 +921,86 @@ extern "C" int fdas_plan_get_timing(fdas_plan* pl, fdas_timing* out)
     }
     return FDAS_OK;
 }
+
+// ------------------------------------------------------------------ candidates
+static std::vector<fdas_sift_trial> sift_trials(const fdas_plan* pl)
+{
+    std::vector<fdas_sift_trial> t(pl->trials.size());
+    for (size_t i = 0; i < t.size(); ++i) {
+        t[i].n = pl->trials[i].n;
+        t[i].nbins = pl->trials[i].nbins;
+        t[i].rlo = pl->trials[i].rlo;
+        t[i].dt = pl->p.dt;
+    }
+    return t;
+}
+
+extern "C" int fdas_stage_powcuts(const fdas_plan* pl, int trial, double sigma, float* powcut)
+{
+    if (!pl || !powcut || trial < 0 || trial >= (int)pl->trials.size()) {
+        set_error("fdas_stage_powcuts: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    const TrialInfo& tr = pl->trials[trial];
+    const int stages = h_fast_log2(pl->H) + 1;
+    for (int s = 0; s < FDAS_MAX_STAGES; ++s)
+        powcut[s] = s < stages ? (float)power_for_sigma(sigma, 1 << s,
+                                                        numindep_count(tr.rlo, (double)tr.nbins, pl->Z0, s))
+                               : 0.f;
+    return FDAS_OK;
+}
+
+extern "C" int fdas_sift(const fdas_plan* pl, const fdas_candidate* in, int n_in,
+                         const fdas_sift_params* params, fdas_accelcand* out, int capacity, int* n_out)
+{
+    if (!pl || (!in && n_in > 0) || n_in < 0 || !n_out) {
+        set_error("fdas_sift: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    const std::vector<fdas_sift_trial> t = sift_trials(pl);
+    return sift_candidates(t.data(), (int)t.size(), pl->p.zmax, pl->uselen, pl->Z0, in, n_in, params,
+                           out, capacity, n_out);
+}
+
+extern "C" int fdas_optimize(fdas_plan* pl, fdas_accelcand* c, int n)
+{
+    if (!pl || (!c && n > 0) || n < 0) {
+        set_error("fdas_optimize: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    std::vector<long long> off(pl->trials.size());
+    std::vector<int> nb(pl->trials.size());
+    for (size_t i = 0; i < off.size(); ++i) {
+        off[i] = pl->trials[i].spec_off;
+        nb[i] = (int)pl->trials[i].nbins;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (c[i].trial < 0 || c[i].trial >= (int)pl->trials.size() || c[i].numharm < 1 ||
+            c[i].numharm > MAX_HARM) {
+            set_error("fdas_optimize: candidate %d has trial %d / numharm %d out of range", i,
+                      c[i].trial, c[i].numharm);
+            return FDAS_ERR_INVALID;
+        }
+    }
+    FDAS_TRY(optimize_candidates(pl->d_spec, off, nb, c, n, pl->stream));
+    for (int i = 0; i < n; ++i) {
+        const TrialInfo& tr = pl->trials[c[i].trial];
+        c[i].opt_sigma = (float)candidate_sigma(
+            c[i].opt_power, c[i].numharm,
+            numindep_count(tr.rlo, (double)tr.nbins, pl->Z0, h_fast_log2(c[i].numharm)));
+        cand_properties(&c[i], (double)tr.n * pl->p.dt);
+    }
+    return FDAS_OK;
+}
+
+extern "C" int fdas_rz_power(fdas_plan* pl, int trial, const double* r, const double* z, int count,
+                             float* out)
+{
+    if (!pl || trial < 0 || trial >= (int)pl->trials.size() || count < 0 ||
+        (count > 0 && (!r || !z || !out))) {
+        set_error("fdas_rz_power: bad argument");
+        return FDAS_ERR_INVALID;
+    }
+    const TrialInfo& tr = pl->trials[trial];
+    return rz_powers(pl->d_spec + tr.spec_off, (int)tr.nbins, r, z, count, out, pl->stream);
+}

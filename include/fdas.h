@@ -267,6 +267,72 @@ int fdas_plan_enable_timing(fdas_plan *plan, int enable);
 int fdas_plan_get_timing(fdas_plan *plan, fdas_timing *out);
 
 /* ------------------------------------------------------------------------ */
+/* Candidates: significance, sifting, optimisation, properties               */
+/* ------------------------------------------------------------------------ */
+/* Replace the reference's empty search_ffdot_powers / sortCandidates /
+ * eliminateHarmonics / optimizeCandidates / calculateProperties
+ * (headers/bpd_candidates.cuh:15-49, source/bpd_candidates.cu:4-33) and its unset
+ * powersCutoff / independentSpectraCount (AccelSearch.cu:37-38,95-97).        */
+
+/* Equivalent Gaussian sigma of a summed power of numharm harmonics (Gamma(numharm,1)
+ * tail) over numindep independent trials, and its inverse. */
+double fdas_candidate_sigma(double power, int numharm, double numindep);
+double fdas_power_for_sigma(double sigma, int numharm, double numindep);
+/* Independent trials of the stage summing 2^stage harmonics over [rlo, rhi) bins
+ * and numz drifts: (rhi-rlo)(numz+1)(2/6.95)/2^stage ((rhi-rlo)/2^stage if numz 1). */
+double fdas_numindep(double rlo, double rhi, int numz, int stage);
+/* powcut[s] = power whose sigma is `sigma` at stage s, for one trial of a plan. */
+int fdas_stage_powcuts(const fdas_plan *plan, int trial, double sigma, float powcut[FDAS_MAX_STAGES]);
+
+typedef struct fdas_accelcand {
+    int trial;
+    int numharm;                 /* harmonics summed at detection (2^stage)        */
+    int step, z_index, r_index, stage;   /* search cell of the detection           */
+    float power;                 /* summed normalised power at that cell            */
+    float sigma;                 /* equivalent Gaussian sigma (stage numindep)      */
+    double r, z;                 /* first harmonic: Fourier frequency, drift (bins) */
+    double opt_r, opt_z;         /* after fdas_optimize (else r, z)                 */
+    float opt_power, opt_sigma;  /* summed power / sigma at (opt_r, opt_z)          */
+    float harm_power[FDAS_MAX_HARM]; /* per harmonic at the optimum                 */
+    double freq, fdot;           /* Hz, Hz/s at (opt_r, opt_z): r/T, z/T^2          */
+    double period, pdot;         /* s, s/s                                          */
+} fdas_accelcand;
+
+typedef struct fdas_sift_params {
+    double sigma_min;   /* drop candidates below this sigma                         */
+    double r_tol;       /* first-harmonic bins for duplicates/harmonics (0: 1.1)    */
+    double z_tol;       /* first-harmonic drift bins (0: 4)                         */
+    int max_ratio;      /* harmonic ratios n/d, n,d <= max_ratio (0: 16, <0: none)  */
+} fdas_sift_params;
+
+typedef struct fdas_sift_trial {
+    long n;             /* series length                                          */
+    long nbins;         /* searched bins: n/2                                     */
+    int rlo;
+    float dt;
+} fdas_sift_trial;
+
+/* Search candidates -> first-harmonic candidates with sigma, sorted by sigma
+ * (descending); duplicates (|dr| <= r_tol, |dz| <= z_tol, any stage) and n/d
+ * harmonics of stronger candidates removed.  *n_out = number kept (the first
+ * min(capacity, *n_out) are written). */
+int fdas_sift(const fdas_plan *plan, const fdas_candidate *in, int n_in,
+              const fdas_sift_params *params, fdas_accelcand *out, int capacity, int *n_out);
+/* The same on explicit trial geometry (no plan, no device). */
+int fdas_sift_raw(const fdas_sift_trial *trials, int ntrials, int zmax, int uselen, int numz,
+                  const fdas_candidate *in, int n_in, const fdas_sift_params *params,
+                  fdas_accelcand *out, int capacity, int *n_out);
+/* Maximise the summed power of each candidate's harmonics over continuous (r, z)
+ * on the plan's whitened spectra (grid refinement on the GPU, FP64 response),
+ * then fill opt_*, harm_power and the properties. */
+int fdas_optimize(fdas_plan *plan, fdas_accelcand *cands, int n);
+/* |sum_j a_j conj(R(r - z/2 - j, z))|^2, j in [r - hw(z), r + hw(z)), on a trial's
+ * whitened spectrum (no normalisation) at count points, with R(q, z) =
+ * int_0^1 exp(2 pi i (q u + z u^2/2)) du: the response the optimiser maximises. */
+int fdas_rz_power(fdas_plan *plan, int trial, const double *r, const double *z, int count,
+                  float *out);
+
+/* ------------------------------------------------------------------------ */
 /* Autotuning                                                                */
 /* ------------------------------------------------------------------------ */
 /* Measures, on this device and synthetic data: (1) every launch shape of the

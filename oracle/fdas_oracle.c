@@ -718,3 +718,68 @@ void or_step_plane(const or_ctx *c, int step, double *out, double *stage_out)
     free(rowbuf);
     free(amps);
 }
+
+/* ------------------------------------------------------------------ optimiser
+ * n-point Gauss-Legendre nodes on [-1, 1] by Newton iteration on P_n. */
+static void or_gauss_legendre(int n, double *x, double *w)
+{
+    for (int i = 0; i < (n + 1) / 2; i++) {
+        double t = cos(OR_PI * (i + 0.75) / (n + 0.5)), pp = 0.0;
+        for (int it = 0; it < 100; it++) {
+            double p0 = 1.0, p1 = t;
+            for (int k = 2; k <= n; k++) {
+                double p2 = ((2.0 * k - 1.0) * t * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            pp = n * (t * p1 - p0) / (t * t - 1.0);
+            double dt = p1 / pp;
+            t -= dt;
+            if (fabs(dt) < 1e-16) break;
+        }
+        x[i] = -t;
+        x[n - 1 - i] = t;
+        w[i] = w[n - 1 - i] = 2.0 / ((1.0 - t * t) * pp * pp);
+    }
+}
+
+void or_rz_response(double q, double z, double *re, double *im)
+{
+    static double gx[16], gw[16];
+    static int init = 0;
+    if (!init) {
+        or_gauss_legendre(16, gx, gw);
+        init = 1;
+    }
+    /* phase rate 2 pi (q + z u): about one cycle per panel at most */
+    double cyc = fabs(q) + fabs(z) + 1.0;
+    int panels = (int)ceil(2.0 * cyc);
+    double sr = 0.0, si = 0.0, h = 0.5 / panels;
+    for (int p = 0; p < panels; p++) {
+        double mid = (2 * p + 1) * h;
+        for (int g = 0; g < 16; g++) {
+            double u = mid + gx[g] * h;
+            double ph = 2.0 * OR_PI * (q * u + 0.5 * z * u * u);
+            sr += gw[g] * h * cos(ph);
+            si += gw[g] * h * sin(ph);
+        }
+    }
+    *re = sr;
+    *im = si;
+}
+
+double or_rz_power(const float *spec, long nbins, double r, double z)
+{
+    int hw = or_z_response_halfwidth(z);
+    long j0 = (long)ceil(r - hw), j1 = (long)ceil(r + hw) - 1;   /* [r - hw, r + hw) */
+    double ar = 0.0, ai = 0.0;
+    for (long j = j0; j <= j1; j++) {
+        if (j < 0 || j >= nbins) continue;
+        double rr, ri;
+        or_rz_response(r - 0.5 * z - j, z, &rr, &ri);
+        double xr = spec[2 * j], xi = spec[2 * j + 1];
+        ar += xr * rr + xi * ri;
+        ai += xi * rr - xr * ri;
+    }
+    return ar * ar + ai * ai;
+}

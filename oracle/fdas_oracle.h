@@ -100,6 +100,14 @@ const float *or_ctx_spectrum(const or_ctx *c);  /* whitened, nbins complex */
  * per-stage partial-sum planes stage_out[s][z*U+i] (NULL to skip). */
 void    or_step_plane(const or_ctx *c, int step, double *out, double *stage_out);
 
+/* ---- candidate optimiser response (build-defined; the reference's optimiser is
+ * a stub, source/bpd_candidates.cu:22-26).  R(q, z) = int_0^1 exp(2 pi i (q u +
+ * z u^2 / 2)) du by composite Gauss-Legendre quadrature (not the Fresnel form the
+ * templates use, source/bpd_subharmonics.cu:146-207), and the matched power
+ * |sum_j a_j conj(R(r - z/2 - j, z))|^2 over integers j in [r - hw(z), r + hw(z)). */
+void    or_rz_response(double q, double z, double *re, double *im);
+double  or_rz_power(const float *spec, long nbins, double r, double z);
+
 #ifdef __cplusplus
 }
 #endif

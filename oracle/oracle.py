@@ -108,6 +108,11 @@ def lib():
         L.or_ctx_spectrum.argtypes = [C.c_void_p]
         L.or_step_plane.restype = None
         L.or_step_plane.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p]
+        L.or_rz_response.restype = None
+        L.or_rz_response.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+        L.or_rz_power.restype = C.c_double
+        L.or_rz_power.argtypes = [_fp, C.c_long, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -234,3 +239,16 @@ class Search:
             self.close()
         except Exception:
             pass
+
+
+def rz_response(q: float, z: float) -> complex:
+    """int_0^1 exp(2 pi i (q u + z u^2/2)) du by Gauss-Legendre quadrature."""
+    re, im = C.c_double(), C.c_double()
+    lib().or_rz_response(float(q), float(z), C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def rz_power(spec: np.ndarray, r: float, z: float) -> float:
+    """Matched power of a complex64 spectrum at (r, z) (no normalisation)."""
+    s = np.ascontiguousarray(spec, dtype=np.complex64)
+    return lib().or_rz_power(s.view(np.float32), len(s), float(r), float(z))

@@ -86,3 +86,22 @@ def test_template_row_layout(oracle, z, L):
     np.testing.assert_array_equal(row[:half], f32(taps[half:]))
     np.testing.assert_array_equal(row[L - half:], f32(taps[:half]))
     assert np.all(row[half:L - half] == 0)
+
+
+def test_rz_response_quadrature(oracle):
+    """The optimiser's response R(q, z) = int_0^1 exp(2 pi i (q u + z u^2/2)) du:
+    the oracle's Gauss-Legendre quadrature against mpmath, and against the
+    Fresnel-form template taps (tap i = R(numkern/4 - z/2 - i/2, z))."""
+    import mpmath as mp
+    for q, z in ((0.3, 0.0), (2.7, -3.0), (-10.2, 40.0), (5.5, 150.5), (-300.2, 1100.0)):
+        v = oracle.rz_response(q, z)
+        w = complex(mp.quad(lambda u: mp.expj(2 * mp.pi * (q * u + z * u * u / 2)),
+                            mp.linspace(0, 1, 200)))
+        assert abs(v - w) < 1e-13, (q, z, v, w)
+    for z in (-24.0, 2.0, 100.0, -600.0):
+        hw = oracle.lib().or_z_response_halfwidth(z)
+        nk = 4 * hw
+        taps = oracle.z_response(z, nk)
+        for i in range(0, nk, max(1, nk // 17)):
+            want = oracle.rz_response(nk / 4 - z / 2 - i / 2, z)
+            assert abs(taps[i] - want) < 1e-12, (z, i)
