@@ -111,6 +111,12 @@ class FdasSiftTrial(C.Structure):
     _fields_ = [("n", C.c_long), ("nbins", C.c_long), ("rlo", C.c_int), ("dt", C.c_float)]
 
 
+class FdasInf(C.Structure):
+    _fields_ = [("basename", C.c_char * 256), ("telescope", C.c_char * 64), ("n", C.c_long),
+                ("dt", C.c_double), ("dm", C.c_double), ("epoch_mjd", C.c_double),
+                ("barycentred", C.c_int)]
+
+
 TUNE_MAX = 8
 
 
@@ -182,6 +188,9 @@ PROTOTYPES = [
     ("fdas_batch_spectrum", i32, [vp, i32, vp]),
     ("fdas_plan_enable_timing", i32, [vp, i32]),
     ("fdas_plan_get_timing", i32, [vp, C.POINTER(FdasTiming)]),
+    ("readData", C.c_long, [C.c_char_p, C.POINTER(C.POINTER(C.c_float))]),
+    ("fdas_free_data", None, [C.POINTER(C.c_float)]),
+    ("fdas_read_inf", i32, [C.c_char_p, C.POINTER(FdasInf)]),
     ("fdas_candidate_sigma", C.c_double, [C.c_double, i32, C.c_double]),
     ("fdas_power_for_sigma", C.c_double, [C.c_double, i32, C.c_double]),
     ("fdas_numindep", C.c_double, [C.c_double, C.c_double, i32, i32]),

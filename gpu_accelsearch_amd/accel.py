@@ -460,6 +460,62 @@ def resolve_fftlen(params: SearchParams) -> int:
     return lib().fdas_resolve_fftlen(C.byref(cp))
 
 
+# ----------------------------------------------------------------- input files
+PADDING = 2000   # zero floats around readData's samples (headers/bpd_common.h:35)
+
+
+def readData(path: str) -> np.ndarray:
+    """Samples of a PRESTO .dat file (float32 copy), via the library's readData."""
+    p = C.POINTER(C.c_float)()
+    n = lib().readData(str(path).encode(), C.byref(p))
+    if n < 0:
+        raise N.FdasError(f"readData: {N.last_error()}")
+    try:
+        return np.ctypeslib.as_array(p, shape=(n,)).copy()
+    finally:
+        lib().fdas_free_data(p)
+
+
+def read_inf(path: str) -> dict:
+    """Sample count, sample time, DM, epoch and names of a PRESTO .inf file."""
+    inf = N.FdasInf()
+    check(lib().fdas_read_inf(str(path).encode(), C.byref(inf)), "fdas_read_inf")
+    return {"basename": inf.basename.decode(), "telescope": inf.telescope.decode(), "n": inf.n,
+            "dt": inf.dt, "dm": inf.dm, "epoch_mjd": inf.epoch_mjd,
+            "barycentred": bool(inf.barycentred)}
+
+
+def load_dm_trials(basenames):
+    """(series list, .inf dicts) of DM trials given as paths without suffix
+    (``<base>.dat`` + ``<base>.inf``).  The .dat length must match the .inf."""
+    series, infos = [], []
+    for b in basenames:
+        inf = read_inf(f"{b}.inf")
+        x = readData(f"{b}.dat")
+        if len(x) != inf["n"]:
+            raise ValueError(f"{b}: .dat holds {len(x)} samples, .inf says {inf['n']}")
+        series.append(x)
+        infos.append(inf)
+    return series, infos
+
+
+def batch_from_files(params: SearchParams, basenames, graph: bool = False) -> "BatchSearch":
+    """A BatchSearch over PRESTO DM-trial files (odd lengths lose their last
+    sample: the search needs even lengths).  All trials must share dt, which sets
+    params.dt."""
+    series, infos = load_dm_trials(basenames)
+    dts = {i["dt"] for i in infos}
+    if len(dts) != 1:
+        raise ValueError(f"DM trials with different sample times: {sorted(dts)}")
+    params = SearchParams(**{**params.__dict__, "dt": dts.pop(), "n": 0})
+    lengths = [len(x) - (len(x) & 1) for x in series]
+    b = BatchSearch(params, lengths, graph=graph)
+    for t, (x, n) in enumerate(zip(series, lengths)):
+        b.upload(t, x[:n])
+    b.infos = infos
+    return b
+
+
 def stage_powcut(sigma: float, numharm: int, numindep: float = 1.0) -> float:
     """Summed power whose false-alarm probability over ``numindep`` trials equals
     the Gaussian tail beyond ``sigma``.  Each normalised power is Exp(1) under

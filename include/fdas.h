@@ -421,6 +421,29 @@ int fdas_server_get_stats(fdas_server *srv, fdas_server_stats *out);
 int fdas_server_reset_stats(fdas_server *srv);
 
 /* ------------------------------------------------------------------------ */
+/* Input files                                                               */
+/* ------------------------------------------------------------------------ */
+/* headers/bpd_data.h:13 readData: raw float32 samples (at most 2^26) of a PRESTO
+ * .dat file into a buffer with 2000 zero floats before and after the data;
+ * *output_data points at the first sample.  Returns the sample count, or -1
+ * with fdas_last_error set (the reference exits).  Free with fdas_free_data. */
+long readData(char *path, float **output_data);
+void fdas_free_data(float *data);
+
+/* PRESTO .inf metadata; the reference's driver hard-codes the sample time
+ * (AccelSearch.cu:90). */
+typedef struct fdas_inf {
+    char basename[256];
+    char telescope[64];
+    long n;              /* Number of bins in the time series      */
+    double dt;           /* Width of each time series bin (sec)    */
+    double dm;           /* Dispersion measure (cm-3 pc)           */
+    double epoch_mjd;    /* Epoch of observation (MJD)             */
+    int barycentred;
+} fdas_inf;
+int fdas_read_inf(const char *path, fdas_inf *out);
+
+/* ------------------------------------------------------------------------ */
 /* Memory plumbing and diagnostics                                           */
 /* ------------------------------------------------------------------------ */
 int fdas_malloc(void **d_ptr, size_t bytes);

@@ -169,3 +169,33 @@ def test_batch_workload_numharm16_autotuned(fd, oracle):
             assert len(exp) > 0 and mine - amb == exp - amb
             ref.close()
         assert {int(c["trial"]) for c in cands} >= set(with_pulsar)
+
+
+def test_batch_from_presto_files(fd, tmp_path):
+    """DM trials read from .dat/.inf files search exactly like in-memory arrays."""
+    from gpu_accelsearch_amd import synth
+    lengths = [300_001, 262_144, 400_000]
+    bases = []
+    for t, n in enumerate(lengths):
+        x = synth.plain_series(n, seed=60 + t, r0=8000.3 + 900 * t, z0=6.0, power=90.0)
+        base = tmp_path / f"trial_DM{t}.00"
+        x.tofile(f"{base}.dat")
+        with open(f"{base}.inf", "w") as f:
+            f.write(f" Data file name without suffix          =  {base.name}\n"
+                    f" Number of bins in the time series      =  {n}\n"
+                    f" Width of each time series bin (sec)    =  6.4e-05\n"
+                    f" Dispersion measure (cm-3 pc)           =  {t}.0\n")
+        bases.append(str(base))
+    P = fd.SearchParams(n=0, zmax=50, numharm=4, fftlen=2048, powcut=POWCUT)
+    print_case("batch from files")
+    with fd.batch_from_files(P, bases) as b:
+        assert [tr.n for tr in b.trials] == [300_000, 262_144, 400_000]
+        assert [i["dm"] for i in b.infos] == [0.0, 1.0, 2.0]
+        b.run()
+        got = _cand_set(b.candidates())
+    xs = [fd.readData(f"{base}.dat")[:n - (n & 1)] for base, n in zip(bases, lengths)]
+    with fd.BatchSearch(P, [len(x) for x in xs]) as b2:
+        for t, x in enumerate(xs):
+            b2.upload(t, x)
+        b2.run()
+        assert got == _cand_set(b2.candidates()) and len(got) > 0
