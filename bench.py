@@ -322,7 +322,8 @@ def main() -> int:
         "kernels": kern,
         "candidates": int(len(cands)),
         "autotune": None if tune is None else
-        {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"]} for k, v in tune["corr"].items()},
+        {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"], "split": v["split"]}
+         for k, v in tune["corr"].items()},
     }
     spec = s.spectrum() if args.stage_steps > 0 else None
     s.close()

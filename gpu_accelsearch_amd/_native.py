@@ -118,6 +118,7 @@ class FdasInf(C.Structure):
 
 
 TUNE_MAX = 8
+TUNE_MAXV = 8
 
 
 class FdasTuneReport(C.Structure):
@@ -125,7 +126,8 @@ class FdasTuneReport(C.Structure):
                 ("cand_fftlen", C.c_int * TUNE_MAX), ("cand_ms_per_trial", C.c_double * TUNE_MAX),
                 ("ncorr", C.c_int), ("corr_len", C.c_int * TUNE_MAX),
                 ("corr_variant", C.c_int * TUNE_MAX), ("corr_threads", C.c_int * TUNE_MAX),
-                ("corr_steps", C.c_int * TUNE_MAX), ("corr_us", (C.c_double * 4) * TUNE_MAX)]
+                ("corr_steps", C.c_int * TUNE_MAX), ("corr_split", C.c_int * TUNE_MAX),
+                ("corr_us", (C.c_double * TUNE_MAXV) * TUNE_MAX)]
 
 
 vp = C.c_void_p

@@ -448,7 +448,7 @@ def autotune(params: SearchParams, fftlens=()) -> dict:
         "ms_per_trial": {rep.cand_fftlen[i]: (rep.cand_ms_per_trial[i] if rep.cand_ms_per_trial[i] >= 0
                                               else None) for i in range(rep.ncand)},
         "corr": {rep.corr_len[i]: {"variant": rep.corr_variant[i], "threads": rep.corr_threads[i],
-                                   "steps_per_wg": rep.corr_steps[i],
+                                   "steps_per_wg": rep.corr_steps[i], "split": bool(rep.corr_split[i]),
                                    "us_per_variant": [u for u in rep.corr_us[i]
                                                       [:lib().fdas_corr_variant_count(rep.corr_len[i])]]}
                  for i in range(rep.ncorr)},

@@ -121,12 +121,11 @@ struct HarmSumArgs {
     int lm[MAX_HARM];             // log2 of the denominator
     int zlo[MAX_HARM];
     int Z[MAX_HARM];
-    int W[MAX_HARM];              // sub-plane row width
+    int W[MAX_HARM];              // sub-plane row width (a multiple of 4 for h >= 1)
     long long sub_off[MAX_HARM];  // element offset of sub-plane h in the sub buffer
     long long sub_stride[MAX_HARM];  // elements per step (Z*W)
     float powcut[5];
     int cand_cap;
-    int sub_half;                 // sub-harmonic planes stored as FP16
 };
 
 // Normalise + transform every segment in desc[0..nseg) (all of FFT length L),
@@ -151,6 +150,8 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s);
 struct CorrArgs {
     const void* bank;
     const int* bank_exp;
+    const void* bank_split;      // split-form rows [E | O] (split variants), same format
+    const int* bank_split_exp;
     int Z;
     const void* seg;
     long long seg_stride;
@@ -169,14 +170,16 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant = -1);
 int corr_variant_count(int L);
 int corr_variant(int L);
 int set_corr_variant(int L, int v);
-void corr_variant_shape(int L, int v, int* threads, int* steps);
+void corr_variant_shape(int L, int v, int* threads, int* steps, int* split = nullptr);
+// Split-form rows of a full FP32 bank (Z rows of length L).
+int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t s);
 // Reduced-precision copy of a bank of Z rows of length L.
 int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
                      hipStream_t s);
 
 // Harmonic sum in place over d_plane[step][Z0][U] (fundamental powers on entry),
 // stage thresholds and candidate emission.
-int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s);
 

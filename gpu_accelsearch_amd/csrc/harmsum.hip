@@ -9,12 +9,15 @@
 // One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step (2 x 2048).  The cells
 // of sub-harmonic h that the tile reads form a rectangle of about (HS_RJ f + 1) x
 // (HS_CB f + 1) values; the rectangles of as many harmonics as fit are staged in
-// LDS with coalesced loads (all loads of a harmonic issued before any store), and
-// the sum gathers from LDS.  Each fundamental cell thus costs about sum_h f_h^2
-// distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1 scattered ones.
+// LDS with 16-byte loads and stores (rows start on 4-float boundaries: sub-plane
+// rows are padded to multiples of 4), all loads of a round issued before any
+// store, and the sum gathers from LDS.  Each fundamental cell thus costs about
+// sum_h f_h^2 distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1
+// scattered ones.
 #include <cstdlib>
 
 #include "fdas_internal.hpp"
+#include "harmidx.hpp"
 #include "precision.hpp"
 
 namespace fdas {
@@ -24,42 +27,14 @@ namespace fdas {
 #define HS_TILE_CONSTANTS                                                      \
     constexpr int HS_CPT = HS_CB / HS_T;                                       \
     constexpr int HS_MAXR = HS_RJ + 1;                                         \
-    constexpr int HS_MAXK = (HS_CB + 2 + HS_T - 1) / HS_T;                     \
-    static_assert(HS_MAXR * (HS_CB + 2) <= HS_LDS, "one rectangle must fit");
-
-__device__ __forceinline__ int hs_rind(unsigned q, int c, int lm, int rlo2)
-{
-    return (int)((q * (unsigned)c + (1u << (lm - 1))) >> lm) - rlo2;
-}
-
-__device__ __forceinline__ int hs_zind(int zz, int c, int lm, int zlo_h)
-{
-    const int x = zz * c;
-    const int subz = x >= 0 ? (x >> lm) : -((-x) >> lm);
-    return (subz - zlo_h) >> 1;
-}
-
-__device__ __forceinline__ void hs_emit(fdas_candidate* cands, int* ncand, int cap, int step,
-                                        int z, int col, int stage, float p)
-{
-    const int idx = atomicAdd(ncand, 1);
-    if (idx < cap) {
-        fdas_candidate c;
-        c.step = step;
-        c.z_index = z;
-        c.r_index = col;
-        c.stage = stage;
-        c.power = p;
-        c.trial = 0;
-        cands[idx] = c;
-    }
-}
+    constexpr int HS_MAXQ = ((HS_CB + 8) / 4 + HS_T - 1) / HS_T;               \
+    static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
 // fundamental powers on entry and the summed powers on exit.
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename SubE>
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
 __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
-                                                       const SubE* __restrict__ sub,
+                                                       const float* __restrict__ sub,
                                                        HarmSumArgs hs, int step0, int Z0,
                                                        const int* __restrict__ fullrlo,
                                                        const int* __restrict__ rlo_h,
@@ -67,7 +42,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                                                        int* __restrict__ ncand)
 {
     HS_TILE_CONSTANTS
-    __shared__ float tile[HS_LDS];
+    __shared__ __align__(16) float tile[HS_LDS];
     const int t = threadIdx.x;
     const int c0 = blockIdx.x * HS_CB, j0 = blockIdx.y * HS_RJ, s = blockIdx.z;
     const int gstep = step0 + s;
@@ -106,15 +81,17 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
 
     int stage = 0;
     for (int h = 1; h < hs.H;) {
-        // harmonics [h, h2) whose rectangles fit in LDS together
+        // harmonics [h, h2) whose rectangles fit in LDS together; a rectangle row
+        // holds columns [ri0 & ~3, ...) rounded up to a multiple of 4
         int h2 = h, used = 0;
         while (h2 < hs.H) {
             const int c = hs.c[h2], lm = hs.lm[h2], r2 = 2 * rl[h2];
             const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[h2]) -
                            hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[h2]) + 1;
-            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - hs_rind(q0 + c0, c, lm, r2) + 1;
-            if (used + nr * nc > HS_LDS) break;
-            used += nr * nc;
+            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
+            const int nca = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - (ri0 & ~3) + 3) & ~3;
+            if (used + nr * nca > HS_LDS) break;
+            used += nr * nca;
             ++h2;
         }
         __syncthreads();   // earlier readers of the tile are done
@@ -123,30 +100,32 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
             const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
             const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
             const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
-            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
-            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0 + 1;
-            const SubE* src = sub + hs.sub_off[g] + s * hs.sub_stride[g] +
-                               (long long)zi0 * hs.W[g] + ri0;
-            float v[HS_MAXR][HS_MAXK];
+            const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
+            const int nq = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) >> 2;
+            const float4* src = reinterpret_cast<const float4*>(
+                sub + hs.sub_off[g] + s * hs.sub_stride[g] + (long long)zi0 * hs.W[g] + ri0a);
+            const int wq = hs.W[g] >> 2;
+            float4 v[HS_MAXR][HS_MAXQ];
 #pragma unroll
             for (int rr = 0; rr < HS_MAXR; ++rr)
 #pragma unroll
-                for (int k = 0; k < HS_MAXK; ++k) {
-                    const int col = t + k * HS_T;
-                    v[rr][k] = (rr < nr && col < nc) ? PStore<SubE>::get(src[(long long)rr * hs.W[g] + col]) : 0.f;
+                for (int k = 0; k < HS_MAXQ; ++k) {
+                    const int q = t + k * HS_T;
+                    v[rr][k] = (rr < nr && q < nq) ? src[(long long)rr * wq + q] : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
             if (first) {
                 mark_fundamental();
                 first = false;
             }
+            float4* dst = reinterpret_cast<float4*>(tile + base);
 #pragma unroll
             for (int rr = 0; rr < HS_MAXR; ++rr)
 #pragma unroll
-                for (int k = 0; k < HS_MAXK; ++k) {
-                    const int col = t + k * HS_T;
-                    if (rr < nr && col < nc) tile[base + rr * nc + col] = v[rr][k];
+                for (int k = 0; k < HS_MAXQ; ++k) {
+                    const int q = t + k * HS_T;
+                    if (rr < nr && q < nq) dst[rr * nq + q] = v[rr][k];
                 }
-            base += nr * nc;
+            base += nr * nq * 4;
         }
         __syncthreads();
         base = 0;
@@ -154,21 +133,21 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
             const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
             const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
             const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
-            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
-            const int nc = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0 + 1;
+            const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
+            const int nca = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) & ~3;
             int ri[HS_CPT];
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
                 const int col = min(c0 + t + k * HS_T, c0 + ncol - 1);
-                ri[k] = hs_rind(q0 + col, c, lm, r2) - ri0;
+                ri[k] = hs_rind(q0 + col, c, lm, r2) - ri0a;
             }
 #pragma unroll
             for (int r = 0; r < HS_RJ; ++r) {
                 const int zi = hs_zind(hs.zlo0 + DZ * (j0 + min(r, nrow - 1)), c, lm, hs.zlo[g]) - zi0;
 #pragma unroll
-                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[base + zi * nc + ri[k]];
+                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[base + zi * nca + ri[k]];
             }
-            base += nr * nc;
+            base += nr * nca;
             if (((g + 1) & g) == 0) {
                 ++stage;
                 const float pc = hs.powcut[stage];
@@ -210,7 +189,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                 const int c = hs.c[h], lm = hs.lm[h];
                 const int rind = hs_rind(q0 + col, c, lm, 2 * rl[h]);
                 const int zind = hs_zind(zz, c, lm, hs.zlo[h]);
-                a2 += PStore<SubE>::get(sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind]);
+                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
                 if (((h + 1) & h) == 0) {
                     ++st;
                     if (a2 > hs.powcut[st]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, st, a2);
@@ -220,20 +199,29 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     }
 }
 
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename SubE>
-static int harmsum_t(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0,
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
+static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0,
                      int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
                      fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
 {
+    for (int h = 1; h < hs.H; ++h) {
+        if ((hs.W[h] & 3) || (hs.sub_off[h] & 3) || (hs.sub_stride[h] & 3)) {
+            set_error("harmonic sum: sub-plane %d rows are not 4-float aligned", h);
+            return FDAS_ERR_INVALID;
+        }
+    }
+    if (((uintptr_t)d_sub) & 15) {
+        set_error("harmonic sum: sub-plane buffer is not 16-byte aligned");
+        return FDAS_ERR_INVALID;
+    }
     dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
               (unsigned)nsteps);
-    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS, SubE>), grid, dim3(HS_T), 0, s,
-                       d_plane, static_cast<const SubE*>(d_sub), hs, step0, Z0, d_fullrlo, d_rlo_h,
-                       d_cands, d_ncand);
+    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS>), grid, dim3(HS_T), 0, s,
+                       d_plane, d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand);
     return check_launch("harmsum_kernel");
 }
 
-int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
+int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s)
 {
@@ -241,11 +229,8 @@ int launch_harmsum(float* d_plane, const void* d_sub, const HarmSumArgs& hs, int
     // Tile shape: 2 rows x 2048 columns, 256 threads, 32 KB of LDS measured
     // fastest of the shapes tried (long contiguous sub-harmonic runs matter more
     // than the staged volume).
-    if (hs.sub_half)
-        return harmsum_t<256, 2, 2048, 8192, unsigned short>(d_plane, d_sub, hs, step0, nsteps, Z0,
-                                                             d_fullrlo, d_rlo_h, d_cands, d_ncand, s);
-    return harmsum_t<256, 2, 2048, 8192, float>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo,
-                                                d_rlo_h, d_cands, d_ncand, s);
+    return harmsum_t<256, 2, 2048, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h,
+                                         d_cands, d_ncand, s);
 }
 
 }  // namespace fdas

@@ -27,17 +27,48 @@ namespace fdas {
 struct BankSet {
     int H = 0;
     int L[MAX_HARM]{}, Z[MAX_HARM]{};
-    void* bankq[MAX_HARM]{};   // stored form used by the kernels
-    int* bank_exp[MAX_HARM]{};  // per-row exponents (reduced precision)
+    void* bankq[MAX_HARM]{};      // stored form used by the kernels
+    int* bank_exp[MAX_HARM]{};    // per-row exponents (reduced precision)
+    void* splitq[MAX_HARM]{};     // split-form rows [E | O] for the split correlation
+    int* split_exp[MAX_HARM]{};
     size_t bytes = 0;
     ~BankSet()
     {
         for (int h = 0; h < MAX_HARM; ++h) {
             (void)hipFree(bankq[h]);
             (void)hipFree(bank_exp[h]);
+            (void)hipFree(splitq[h]);
+            (void)hipFree(split_exp[h]);
         }
     }
 };
+
+// Stored form of one FP32 bank: itself (FP32) or a packed copy; frees the FP32
+// bank when packing.
+static int store_bank(cplx* bank, int Z, int L, int prec, hipStream_t s, void** out, int** exps,
+                      size_t* bytes)
+{
+    if (prec == FDAS_PREC_FP32) {
+        *out = bank;
+        *bytes += (size_t)Z * L * sizeof(cplx);
+        return FDAS_OK;
+    }
+    void* q = nullptr;
+    int* e = nullptr;
+    int rc = FDAS_OK;
+    if (hipMalloc(&q, (size_t)Z * L * 4) != hipSuccess || hipMalloc(&e, (size_t)Z * sizeof(int)) != hipSuccess) {
+        set_error("template bank allocation failed");
+        rc = FDAS_ERR_NOMEM;
+    } else {
+        rc = launch_bank_pack(bank, Z, L, prec, q, e, s);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && rc == FDAS_OK) rc = FDAS_ERR_HIP;
+    (void)hipFree(bank);
+    *out = q;
+    *exps = e;
+    *bytes += (size_t)Z * L * 4 + (size_t)Z * sizeof(int);
+    return rc;
+}
 
 using BankKey = std::tuple<int, int, int, int, int, int>;   // device, zmax, numharm, uselen, prec
 static std::mutex g_bank_mu;
@@ -65,29 +96,24 @@ static int build_banks(int zmax, int H, const int* cur, const int* mx, int usele
         const int Z = (zmax_h / DZ) * 2 + 1, L = h_fft_length(uselen, mx[h], cur[h], zmax);
         bs->Z[h] = Z;
         bs->L[h] = L;
-        cplx* bank = nullptr;
+        cplx *bank = nullptr, *split = nullptr;
         FDAS_HIP(hipMalloc(&bank, (size_t)Z * L * sizeof(cplx)));
-        int rc = build_template_bank(bank, zmax_h, Z, L, s);
-        if (rc == FDAS_OK && prec != FDAS_PREC_FP32) {
-            void* q = nullptr;
-            int* e = nullptr;
-            if (hipMalloc(&q, (size_t)Z * L * 4) != hipSuccess ||
-                hipMalloc(&e, (size_t)Z * sizeof(int)) != hipSuccess) {
-                set_error("template bank allocation failed");
-                rc = FDAS_ERR_NOMEM;
-            } else {
-                rc = launch_bank_pack(bank, Z, L, prec, q, e, s);
-            }
-            if (hipStreamSynchronize(s) != hipSuccess && rc == FDAS_OK) rc = FDAS_ERR_HIP;
+        FDAS_TRY(build_template_bank(bank, zmax_h, Z, L, s));
+        if (hipMalloc(&split, (size_t)Z * L * sizeof(cplx)) != hipSuccess) {
             (void)hipFree(bank);
-            bs->bankq[h] = q;
-            bs->bank_exp[h] = e;
-            bs->bytes += (size_t)Z * L * 4 + (size_t)Z * sizeof(int);
-        } else {
-            bs->bankq[h] = bank;
-            bs->bytes += (size_t)Z * L * sizeof(cplx);
+            set_error("template bank allocation failed");
+            return FDAS_ERR_NOMEM;
         }
-        FDAS_TRY(rc);
+        const int rs = launch_bank_split(bank, Z, L, split, s);
+        if (rs != FDAS_OK || hipStreamSynchronize(s) != hipSuccess) {
+            (void)hipFree(bank);
+            (void)hipFree(split);
+            return rs != FDAS_OK ? rs : FDAS_ERR_HIP;
+        }
+        const int r1 = store_bank(bank, Z, L, prec, s, &bs->bankq[h], &bs->bank_exp[h], &bs->bytes);
+        const int r2 = store_bank(split, Z, L, prec, s, &bs->splitq[h], &bs->split_exp[h], &bs->bytes);
+        FDAS_TRY(r1);
+        FDAS_TRY(r2);
     }
     FDAS_HIP(hipStreamSynchronize(s));
     g_banks[key] = bs;
@@ -169,8 +195,9 @@ namespace {
 // Brackets one launch with events when timing is on.
 struct TimedLaunch {
     fdas_plan* pl;
+    hipStream_t st;
     size_t idx = (size_t)-1;
-    TimedLaunch(fdas_plan* p, int cat) : pl(p)
+    TimedLaunch(fdas_plan* p, int cat) : pl(p), st(p->stream)
     {
         if (!pl->timing) return;
         if (pl->nrec == pl->recs.size()) {
@@ -180,11 +207,11 @@ struct TimedLaunch {
         }
         idx = pl->nrec++;
         pl->recs[idx].cat = cat;
-        (void)hipEventRecord(pl->recs[idx].a, pl->stream);
+        (void)hipEventRecord(pl->recs[idx].a, st);
     }
     ~TimedLaunch()
     {
-        if (idx != (size_t)-1) (void)hipEventRecord(pl->recs[idx].b, pl->stream);
+        if (idx != (size_t)-1) (void)hipEventRecord(pl->recs[idx].b, st);
     }
 };
 
@@ -385,6 +412,9 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
     }
     long long sub = 0;
     for (int h = 0; h < H; ++h) {
+        // sub-harmonic rows padded to 4 floats, so the harmonic sum stages them
+        // with 16-byte loads (the extra columns are correlation outputs too)
+        if (h > 0) pl->W[h] = (pl->W[h] + 3) & ~3;
         if (pl->off[h] + pl->W[h] > pl->L[h]) {
             set_error("harmonic %d: %d columns from offset %d exceed FFT length %d", h, pl->W[h],
                       pl->off[h], pl->L[h]);
@@ -422,8 +452,12 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     // chunking: per-item buffer bytes against a device-memory budget
     const size_t per_item = (size_t)pl->seg_stride * pl->cbytes + (size_t)pl->sub_total * sizeof(float) +
                             (size_t)pl->Z0 * U * sizeof(float) + (size_t)H * sizeof(int);
+    // default: as many items per chunk as 24 GiB of chunk buffers hold (a whole
+    // 2^23-sample trial at zmax 200: fewer, larger launches measured fastest, and
+    // overlapping the harmonic sum of one chunk with the correlations of the
+    // next on a second stream measured slower, since both kernels fill every CU)
     int chunk = p.chunk_steps;
-    if (chunk <= 0) chunk = (int)std::max<size_t>(1, ((size_t)16 << 30) / per_item);
+    if (chunk <= 0) chunk = (int)std::max<size_t>(1, ((size_t)24 << 30) / per_item);
     pl->chunk = std::min(chunk, pl->nitems);
 
     // device state (everything above is host-side validation and geometry)
@@ -492,7 +526,6 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     }
     for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
     hs.cand_cap = pl->cand_cap;
-    hs.sub_half = 0;
     FDAS_HIP(hipStreamSynchronize(pl->stream));
     return FDAS_OK;
 }
@@ -516,6 +549,8 @@ int reset_candidates(fdas_plan* pl)
     return check_launch("zero_int_kernel");
 }
 
+// Chunks of items, each: segment preparation per FFT-length class, the
+// correlations of every harmonic, then the harmonic sum.
 int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
 {
     const int U = pl->uselen;
@@ -541,6 +576,8 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
             CorrArgs a{};
             a.bank = bs.bankq[h];
             a.bank_exp = bs.bank_exp[h];
+            a.bank_split = bs.splitq[h];
+            a.bank_split_exp = bs.split_exp[h];
             a.Z = pl->Z[h];
             a.seg = pl->d_seg;
             a.seg_stride = pl->seg_stride;
@@ -566,6 +603,9 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
             TimedLaunch tl(pl, FDAS_K_CORR_FUND);
             FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream));
         }
+        // The harmonic sum as its own tiled kernel: summing inside the fundamental
+        // correlation's epilogue (gathering sub-harmonic cells after each inverse
+        // FFT) measured 1.3-1.5x slower at FFT lengths 4096 and 8192.
         TimedLaunch tl(pl, FDAS_K_HARMSUM);
         FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo, pl->d_rlo_h,
                                 pl->d_cands, pl->d_ncand, pl->stream));

@@ -252,6 +252,82 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     }
 }
 
+// Split form of the same correlation.  The product X * conj(K) has X = [A, A]
+// (period L/2), so with k = k' + (L/2) j the length-L inverse transform splits
+// into two length-L/2 transforms of A (no spread copy):
+//   y[2m]   = IFFT_{L/2}(A * E)[m],  E[k'] = conj K[k'] + conj K[k' + L/2]
+//   y[2m+1] = IFFT_{L/2}(A * O)[m],  O[k'] = (conj K[k'] - conj K[k' + L/2]) e^{2 pi i k'/L}
+// E and O are precomputed per template row (split bank: row = [E | O]).  Half-
+// length transforms need fewer registers per value, so more waves fit per SIMD.
+template <int L, int G, int T, int P, typename OutE>
+__global__ __launch_bounds__(T) void corr_split_kernel(
+    const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
+    const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
+    const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
+    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale)
+{
+    constexpr int LH = L / 2;
+    using F = BlockFFT<LH, T, 1>;
+    using S = typename CStore<P>::T;
+    constexpr int E = F::E;
+    __shared__ cplx lds[F::LDS_ELEMS];
+    const int z = blockIdx.x, t = threadIdx.x;
+    const int s0 = blockIdx.y * G;
+    const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
+    const S* Krow = bank + (size_t)z * L;
+    [[maybe_unused]] S ke[G > 1 ? E : 1], ko[G > 1 ? E : 1];
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            ke[i] = Krow[F::in_index(t, i)];
+            ko[i] = Krow[LH + F::in_index(t, i)];
+        }
+    }
+    const int ek = P == PREC_F32 ? 0 : bank_exp[z];
+    for (int s = s0; s < s1; ++s) {
+        const S* A = seg + s * seg_stride + seg_off;
+        cplx a[E], v[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) a[i] = CStore<P>::get(A[F::in_index(t, i)]);
+        const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
+        OutE* o = out + ((size_t)s * Z + z) * W;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                S k;
+                if constexpr (G > 1) k = half ? ko[i] : ke[i];
+                else k = Krow[half * LH + F::in_index(t, i)];
+                v[i] = cmul(a[i], CStore<P>::get(k));
+            }
+            F::run(v, lds, tw, t);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const int col = 2 * F::out_index(t, i) + half - off;
+                if (col >= 0 && col < W)
+                    o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
+            }
+        }
+    }
+}
+
+// Split bank rows (see corr_split_kernel) from full rows of length L.
+__global__ __launch_bounds__(256) void bank_split_kernel(const cplx* __restrict__ in, int L,
+                                                         cplx* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int z = blockIdx.y;
+    const int LH = L / 2;
+    if (k >= LH) return;
+    const cplx* row = in + (size_t)z * L;
+    const cplx lo = mk(row[k].x, -row[k].y), hi = mk(row[k + LH].x, -row[k + LH].y);
+    double sw, cw;
+    sincospi(2.0 * (double)k / (double)L, &sw, &cw);
+    const cplx d = mk(lo.x - hi.x, lo.y - hi.y);
+    out[(size_t)z * L + k] = mk(lo.x + hi.x, lo.y + hi.y);
+    out[(size_t)z * L + LH + k] = cmul(d, mk((float)cw, (float)sw));
+}
+
 // Reduced-precision copy of a template bank: per row the exponent and the packed
 // values (one workgroup per row).
 template <int P>
@@ -324,15 +400,15 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // transform itself, so every step reloads its template row.
 namespace {
 struct CorrShape {
-    int T, G;
+    int T, G, S;   // threads, steps per workgroup, split form (two L/2 transforms)
 };
-constexpr CorrShape kShapes256[] = {{64, 8}};
-constexpr CorrShape kShapes512[] = {{64, 8}};
-constexpr CorrShape kShapes1024[] = {{64, 8}, {128, 8}};
-constexpr CorrShape kShapes2048[] = {{128, 8}, {256, 8}};
-constexpr CorrShape kShapes4096[] = {{512, 8}, {256, 8}, {512, 4}};
-constexpr CorrShape kShapes8192[] = {{512, 8}, {512, 4}, {1024, 8}};
-constexpr CorrShape kShapes16384[] = {{512, 1}, {1024, 1}};
+constexpr CorrShape kShapes256[] = {{64, 8, 0}};
+constexpr CorrShape kShapes512[] = {{64, 8, 0}};
+constexpr CorrShape kShapes1024[] = {{64, 8, 0}, {128, 8, 0}};
+constexpr CorrShape kShapes2048[] = {{128, 8, 0}, {256, 8, 0}, {128, 8, 1}};
+constexpr CorrShape kShapes4096[] = {{512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1}};
+constexpr CorrShape kShapes8192[] = {{512, 8, 0}, {512, 4, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1}};
+constexpr CorrShape kShapes16384[] = {{512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1}};
 
 int shape_slot(int L)
 {
@@ -394,38 +470,53 @@ int set_corr_variant(int L, int v)
     return FDAS_OK;
 }
 
-void corr_variant_shape(int L, int v, int* threads, int* steps)
+void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
 {
     int c = 0;
     const CorrShape* sh = shapes_of(L, &c);
-    *threads = (sh && v >= 0 && v < c) ? sh[v].T : 0;
-    *steps = (sh && v >= 0 && v < c) ? sh[v].G : 0;
+    const bool ok = sh && v >= 0 && v < c;
+    *threads = ok ? sh[v].T : 0;
+    *steps = ok ? sh[v].G : 0;
+    if (split) *split = ok ? sh[v].S : 0;
 }
 
-template <int L, int T, int G, int P, typename OutE>
+template <int L, int T, int G, int SPLIT, int P, typename OutE>
 static int corr_t(const CorrArgs& a, hipStream_t s)
 {
     using S = typename CStore<P>::T;
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles_t(L, T, &err);
+    const cplx* tw = fft_twiddles_t(SPLIT ? L / 2 : L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), dim3(a.Z, (a.nsteps + G - 1) / G),
-                       dim3(T), 0, s, static_cast<const S*>(a.bank), a.bank_exp, a.Z,
-                       static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h,
-                       a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
-    return check_launch("corr_sub_kernel");
+    const dim3 grid(a.Z, (a.nsteps + G - 1) / G);
+    if constexpr (SPLIT) {
+        if (!a.bank_split) {
+            set_error("split correlation at length %d without a split bank", L);
+            return FDAS_ERR_INVALID;
+        }
+        hipLaunchKernelGGL((corr_split_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+                           static_cast<const S*>(a.bank_split), a.bank_split_exp, a.Z,
+                           static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H,
+                           a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
+        return check_launch("corr_split_kernel");
+    } else {
+        hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+                           static_cast<const S*>(a.bank), a.bank_exp, a.Z,
+                           static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H,
+                           a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
+        return check_launch("corr_sub_kernel");
+    }
 }
 
-template <int L, int T, int G>
+template <int L, int T, int G, int SPLIT>
 static int corr_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
         case PREC_F16:
-            return a.out_half ? corr_t<L, T, G, PREC_F16, unsigned short>(a, s)
-                              : corr_t<L, T, G, PREC_F16, float>(a, s);
-        case PREC_BF16: return corr_t<L, T, G, PREC_BF16, float>(a, s);
-        default: return corr_t<L, T, G, PREC_F32, float>(a, s);
+            return a.out_half ? corr_t<L, T, G, SPLIT, PREC_F16, unsigned short>(a, s)
+                              : corr_t<L, T, G, SPLIT, PREC_F16, float>(a, s);
+        case PREC_BF16: return corr_t<L, T, G, SPLIT, PREC_BF16, float>(a, s);
+        default: return corr_t<L, T, G, SPLIT, PREC_F32, float>(a, s);
     }
 }
 
@@ -437,9 +528,9 @@ static int corr_l(const CorrArgs& a, hipStream_t s, int v)
     constexpr const CorrShape* SH = K == 0 ? kShapes256 : K == 1 ? kShapes512 : K == 2 ? kShapes1024
                                   : K == 3 ? kShapes2048 : K == 4 ? kShapes4096 : K == 5 ? kShapes8192
                                   : kShapes16384;
-    constexpr int NV = K == 0 ? 1 : K == 1 ? 1 : K == 2 ? 2 : K == 3 ? 2 : K == 4 ? 3 : K == 5 ? 3 : 2;
+    constexpr int NV = K == 0 ? 1 : K == 1 ? 1 : K == 2 ? 2 : K == 3 ? 3 : K == 4 ? 5 : K == 5 ? 5 : 4;
     if constexpr (V < NV) {
-        if (v == V) return corr_p<L, SH[V].T, SH[V].G>(a, s);
+        if (v == V) return corr_p<L, SH[V].T, SH[V].G, SH[V].S>(a, s);
         return corr_l<L, V + 1>(a, s, v);
     } else {
         set_error("correlation variant %d out of range at length %d", v, L);
@@ -454,6 +545,7 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
         set_error("FP16 power planes need FP16 inputs");
         return FDAS_ERR_INVALID;
     }
+
     const int v = variant >= 0 ? variant : corr_variant(L);
     switch (L) {
         case 256: return corr_l<256>(a, s, v);
@@ -483,6 +575,12 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
         return FDAS_ERR_INVALID;
     }
     return check_launch("bank_pack_kernel");
+}
+
+int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t s)
+{
+    hipLaunchKernelGGL(bank_split_kernel, dim3((L / 2 + 255) / 256, Z), dim3(256), 0, s, d_bank, L, d_out);
+    return check_launch("bank_split_kernel");
 }
 
 // ------------------------------------------------------------------ stage path

@@ -130,6 +130,8 @@ int time_corr_variants(int L, int prec, hipStream_t s, std::vector<double>* us)
     CorrArgs ca{};
     ca.bank = bank;
     ca.bank_exp = bexp;
+    ca.bank_split = bank;   // same size and format; contents do not matter for timing
+    ca.bank_split_exp = bexp;
     ca.Z = Z;
     ca.seg = seg;
     ca.seg_stride = L / 2;
@@ -277,13 +279,14 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
             if (rc != FDAS_OK) break;
             int best = 0;
             for (int v = 0; v < (int)us.size(); ++v) {
-                if (v < 4) rep->corr_us[k][v] = us[v];
+                if (v < FDAS_TUNE_MAXV) rep->corr_us[k][v] = us[v];
                 if (us[v] < us[best]) best = v;
             }
             set_corr_variant(L, best);
         }
         rep->corr_variant[k] = corr_variant(L);
-        corr_variant_shape(L, rep->corr_variant[k], &rep->corr_threads[k], &rep->corr_steps[k]);
+        corr_variant_shape(L, rep->corr_variant[k], &rep->corr_threads[k], &rep->corr_steps[k],
+                           &rep->corr_split[k]);
     }
     // 2. whole trials per candidate fundamental length
     int best = 0;

@@ -342,6 +342,7 @@ int fdas_rz_power(fdas_plan *plan, int trial, const double *r, const double *z, 
  * each candidate fundamental FFT length (nfftlen 0: 1024..16384), remembering
  * the fastest for plans with fftlen 0 and the same (zmax, numharm, precision). */
 #define FDAS_TUNE_MAX 8
+#define FDAS_TUNE_MAXV 8
 typedef struct fdas_tune_report {
     int fftlen;                              /* chosen fundamental length        */
     int ncand;
@@ -352,7 +353,8 @@ typedef struct fdas_tune_report {
     int corr_variant[FDAS_TUNE_MAX];         /* selected variant                  */
     int corr_threads[FDAS_TUNE_MAX];
     int corr_steps[FDAS_TUNE_MAX];
-    double corr_us[FDAS_TUNE_MAX][4];        /* us per launch per variant (0: n/a) */
+    int corr_split[FDAS_TUNE_MAX];           /* 1: two L/2 transforms per step    */
+    double corr_us[FDAS_TUNE_MAX][FDAS_TUNE_MAXV]; /* us per launch per variant (0: n/a) */
 } fdas_tune_report;
 
 int fdas_autotune(const fdas_params *base, const int *fftlens, int nfftlen,
