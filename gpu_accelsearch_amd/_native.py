@@ -127,7 +127,10 @@ class FdasTuneReport(C.Structure):
                 ("ncorr", C.c_int), ("corr_len", C.c_int * TUNE_MAX),
                 ("corr_variant", C.c_int * TUNE_MAX), ("corr_threads", C.c_int * TUNE_MAX),
                 ("corr_steps", C.c_int * TUNE_MAX), ("corr_split", C.c_int * TUNE_MAX),
-                ("corr_us", (C.c_double * TUNE_MAXV) * TUNE_MAX)]
+                ("corr_us", (C.c_double * TUNE_MAXV) * TUNE_MAX),
+                ("harmsum_variant", C.c_int), ("harmsum_threads", C.c_int),
+                ("harmsum_rows", C.c_int), ("harmsum_cols", C.c_int),
+                ("harmsum_ms", C.c_double * TUNE_MAXV)]
 
 
 vp = C.c_void_p
@@ -204,6 +207,9 @@ PROTOTYPES = [
     ("fdas_rz_power", i32, [vp, i32, vp, vp, i32, vp]),
     ("fdas_autotune", i32, [C.POINTER(FdasParams), C.POINTER(i32), i32, C.POINTER(FdasTuneReport)]),
     ("fdas_corr_variant_count", i32, [i32]),
+    ("fdas_harmsum_variant_count", i32, []),
+    ("fdas_get_harmsum_variant", i32, []),
+    ("fdas_set_harmsum_variant", i32, [i32]),
     ("fdas_get_corr_variant", i32, [i32]),
     ("fdas_set_corr_variant", i32, [i32, i32]),
     ("fdas_resolve_fftlen", i32, [C.POINTER(FdasParams)]),

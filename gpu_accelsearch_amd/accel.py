@@ -447,6 +447,9 @@ def autotune(params: SearchParams, fftlens=()) -> dict:
         "fftlen": rep.fftlen,
         "ms_per_trial": {rep.cand_fftlen[i]: (rep.cand_ms_per_trial[i] if rep.cand_ms_per_trial[i] >= 0
                                               else None) for i in range(rep.ncand)},
+        "harmsum": {"variant": rep.harmsum_variant, "threads": rep.harmsum_threads,
+                    "rows": rep.harmsum_rows, "cols": rep.harmsum_cols,
+                    "ms_per_trial": [m for m in rep.harmsum_ms[:lib().fdas_harmsum_variant_count()]]},
         "corr": {rep.corr_len[i]: {"variant": rep.corr_variant[i], "threads": rep.corr_threads[i],
                                    "steps_per_wg": rep.corr_steps[i], "split": bool(rep.corr_split[i]),
                                    "us_per_variant": [u for u in rep.corr_us[i]

@@ -179,6 +179,10 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
 
 // Harmonic sum in place over d_plane[step][Z0][U] (fundamental powers on entry),
 // stage thresholds and candidate emission.
+int harmsum_variant_count();
+int harmsum_variant();
+int set_harmsum_variant(int v);
+void harmsum_variant_shape(int v, int* threads, int* rows, int* cols);
 int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s);

@@ -14,6 +14,7 @@
 // store, and the sum gathers from LDS.  Each fundamental cell thus costs about
 // sum_h f_h^2 distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1
 // scattered ones.
+#include <atomic>
 #include <cstdlib>
 
 #include "fdas_internal.hpp"
@@ -221,16 +222,56 @@ static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, 
     return check_launch("harmsum_kernel");
 }
 
+// Tile shapes (threads, rows, columns, staged floats).  The default is variant 4
+// (128 threads, 2 x 1024), the fastest measured on the production trial with
+// 16-byte staging; the autotuner times them all.
+namespace {
+struct HsShape {
+    int T, RJ, CB, LDS;
+};
+constexpr HsShape kHsShapes[] = {{256, 2, 2048, 8192}, {256, 4, 1024, 8192}, {256, 2, 2048, 16384},
+                                 {512, 2, 4096, 16384}, {128, 2, 1024, 4096}, {128, 4, 512, 4096},
+                                 {64, 2, 1024, 4096}, {128, 2, 2048, 8192}};
+constexpr int kHsCount = (int)(sizeof(kHsShapes) / sizeof(kHsShapes[0]));
+std::atomic<int> g_hs_variant{4};
+}  // namespace
+
+int harmsum_variant_count() { return kHsCount; }
+int harmsum_variant() { return g_hs_variant.load(); }
+int set_harmsum_variant(int v)
+{
+    if (v < 0 || v >= kHsCount) {
+        set_error("no harmonic-sum variant %d", v);
+        return FDAS_ERR_INVALID;
+    }
+    g_hs_variant.store(v);
+    return FDAS_OK;
+}
+void harmsum_variant_shape(int v, int* threads, int* rows, int* cols)
+{
+    const bool ok = v >= 0 && v < kHsCount;
+    *threads = ok ? kHsShapes[v].T : 0;
+    *rows = ok ? kHsShapes[v].RJ : 0;
+    *cols = ok ? kHsShapes[v].CB : 0;
+}
+
 int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
                    int* d_ncand, hipStream_t s)
 {
     if (nsteps <= 0) return FDAS_OK;
-    // Tile shape: 2 rows x 2048 columns, 256 threads, 32 KB of LDS measured
-    // fastest of the shapes tried (long contiguous sub-harmonic runs matter more
-    // than the staged volume).
-    return harmsum_t<256, 2, 2048, 8192>(d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h,
-                                         d_cands, d_ncand, s);
+#define HS_ARGS d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s
+    switch (g_hs_variant.load()) {
+        case 1: return harmsum_t<256, 4, 1024, 8192>(HS_ARGS);
+        case 2: return harmsum_t<256, 2, 2048, 16384>(HS_ARGS);
+        case 3: return harmsum_t<512, 2, 4096, 16384>(HS_ARGS);
+        case 5: return harmsum_t<128, 4, 512, 4096>(HS_ARGS);
+        case 6: return harmsum_t<64, 2, 1024, 4096>(HS_ARGS);
+        case 7: return harmsum_t<128, 2, 2048, 8192>(HS_ARGS);
+        case 0: return harmsum_t<256, 2, 2048, 8192>(HS_ARGS);
+        default: return harmsum_t<128, 2, 1024, 4096>(HS_ARGS);
+    }
+#undef HS_ARGS
 }
 
 }  // namespace fdas

@@ -1,11 +1,12 @@
 // tune.hip -- measured autotuning of the search's shapes (fdas_autotune).
 //
-// Two choices are measured on the device, on synthetic data of the caller's
+// Three choices are measured on the device, on synthetic data of the caller's
 // shape:
 //  * the launch shape of the fused correlation per FFT length (threads per
 //    workgroup, which fixes the radix of the LDS passes, and steps per workgroup,
 //    the template-block tile): each variant is timed on a block of 64 templates x
 //    64 steps and the fastest is selected process-wide;
+//  * the tile shape of the harmonic sum: whole trials per shape;
 //  * the fundamental FFT length (overlap-save segment length): whole DM trials
 //    of the caller's series length are timed for every candidate length, and the
 //    fastest is remembered for plans of the same (zmax, numharm, precision) that
@@ -288,7 +289,31 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
         corr_variant_shape(L, rep->corr_variant[k], &rep->corr_threads[k], &rep->corr_steps[k],
                            &rep->corr_split[k]);
     }
-    // 2. whole trials per candidate fundamental length
+    // 2. harmonic-sum tile shape: whole trials at the first valid candidate length
+    for (size_t i = 0; rc == FDAS_OK && i < cand.size() && base->numharm > 1; ++i) {
+        if (!config_valid(cand[i], base->zmax, base->numharm)) continue;
+        int bestv = harmsum_variant();
+        double bms = 0.0;
+        for (int v = 0; v < harmsum_variant_count() && v < FDAS_TUNE_MAXV; ++v) {
+            set_harmsum_variant(v);
+            double ms = 0.0;
+            const int r = time_trial(*base, cand[i], s, &ms);
+            if (r != FDAS_OK) {
+                if (r != FDAS_ERR_INVALID && r != FDAS_ERR_UNSUPPORTED) rc = r;
+                break;
+            }
+            rep->harmsum_ms[v] = ms;
+            if (v == 0 || ms < bms) {
+                bms = ms;
+                bestv = v;
+            }
+        }
+        set_harmsum_variant(bestv);
+        rep->harmsum_variant = bestv;
+        harmsum_variant_shape(bestv, &rep->harmsum_threads, &rep->harmsum_rows, &rep->harmsum_cols);
+        break;
+    }
+    // 3. whole trials per candidate fundamental length
     int best = 0;
     double best_ms = 0.0;
     for (size_t i = 0; rc == FDAS_OK && i < cand.size(); ++i) {
@@ -321,3 +346,7 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
     g_tuned_fftlen[{base->zmax, base->numharm, base->precision}] = best;
     return FDAS_OK;
 }
+
+extern "C" int fdas_harmsum_variant_count(void) { return harmsum_variant_count(); }
+extern "C" int fdas_get_harmsum_variant(void) { return harmsum_variant(); }
+extern "C" int fdas_set_harmsum_variant(int variant) { return set_harmsum_variant(variant); }

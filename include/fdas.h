@@ -338,7 +338,8 @@ int fdas_rz_power(fdas_plan *plan, int trial, const double *r, const double *z, 
 /* Measures, on this device and synthetic data: (1) every launch shape of the
  * fused correlation (threads per workgroup = radix of the LDS passes, steps per
  * workgroup = template-block tile) at each FFT length the candidates use, and
- * selects the fastest process-wide; (2) whole DM trials of base->n samples at
+ * selects the fastest process-wide; (2) the harmonic-sum tile shape over whole
+ * trials at the first valid candidate length; (3) whole DM trials of base->n samples at
  * each candidate fundamental FFT length (nfftlen 0: 1024..16384), remembering
  * the fastest for plans with fftlen 0 and the same (zmax, numharm, precision). */
 #define FDAS_TUNE_MAX 8
@@ -355,11 +356,17 @@ typedef struct fdas_tune_report {
     int corr_steps[FDAS_TUNE_MAX];
     int corr_split[FDAS_TUNE_MAX];           /* 1: two L/2 transforms per step    */
     double corr_us[FDAS_TUNE_MAX][FDAS_TUNE_MAXV]; /* us per launch per variant (0: n/a) */
+    int harmsum_variant;                     /* selected harmonic-sum tile shape   */
+    int harmsum_threads, harmsum_rows, harmsum_cols;
+    double harmsum_ms[FDAS_TUNE_MAXV];       /* ms per trial per shape (0: n/a)    */
 } fdas_tune_report;
 
 int fdas_autotune(const fdas_params *base, const int *fftlens, int nfftlen,
                   fdas_tune_report *report);
 int fdas_corr_variant_count(int fftlen);
+int fdas_harmsum_variant_count(void);
+int fdas_get_harmsum_variant(void);
+int fdas_set_harmsum_variant(int variant);
 int fdas_get_corr_variant(int fftlen);
 int fdas_set_corr_variant(int fftlen, int variant);
 /* The fundamental FFT length a plan with these parameters would use. */

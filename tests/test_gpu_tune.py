@@ -68,3 +68,32 @@ def test_autotune_selects_and_applies(fd):
             assert us[c["variant"]] == min(us)
     with fd.AccelSearch(p) as s:
         assert s.info.fftlen[0] == rep["fftlen"]
+
+
+def test_every_harmsum_shape_is_bit_identical(fd):
+    """The harmonic sum adds harmonics per cell in the same order whatever the
+    tile shape, so every shape gives the same bits (planes and candidates)."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 20
+    x = synth.plain_series(n, seed=23, r0=31000.3, z0=-12.0, power=90.0)
+    L = fd.lib()
+    P = fd.SearchParams(n=n, zmax=100, numharm=16, fftlen=4096, powcut=(30.0, 35.0, 45.0, 60.0, 80.0))
+    ref = None
+    try:
+        with fd.AccelSearch(P) as s:
+            s.prepare(x)
+            for v in range(L.fdas_harmsum_variant_count()):
+                assert L.fdas_set_harmsum_variant(v) == 0
+                print_case("harmsum shape", v)
+                planes = s.run_steps(0, s.nsteps)
+                s.search(x)
+                cands = s.candidates()
+                key = (planes.view(np.uint32), sorted(map(tuple, cands.tolist())))
+                if ref is None:
+                    ref = key
+                    assert len(key[1]) > 0
+                else:
+                    np.testing.assert_array_equal(key[0], ref[0])
+                    assert key[1] == ref[1]
+    finally:
+        L.fdas_set_harmsum_variant(4)
