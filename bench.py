@@ -4,7 +4,9 @@
 A "step" of this benchmark is one DM trial of the production workload:
 2^23-sample float32 series (N(0,1) + red noise + 3 accelerated pulsars, already
 resident in HBM), R2C FFT, whitening, every overlap-save step (1084 x 201 x 7734
-f-fdot cells) with 8-harmonic summing, stage thresholds and the summed plane.
+f-fdot cells), every cell summed over 8 harmonics and tested against the stage
+thresholds, candidates out (fdas_search; summed planes are stored only when a
+caller asks for them with fdas_run_steps).
 
 Prints one JSON line: cells/s (value), DM-trials/s, p50/p99 trial latency, the
 roofline of the dominant kernel (HIP events on the plan's stream), and the CPU
@@ -35,7 +37,8 @@ def alg_bytes(info, n: int) -> dict:
     Templates are resident constants and not counted.
     corr_sub:  segment reads + sub-plane writes.
     corr_fund: fundamental segment reads + fundamental power writes.
-    harmsum:   sub-plane reads + fundamental plane read + summed plane write."""
+    harmsum:   sub-plane reads + fundamental plane read (the summed plane is
+               not stored by fdas_search)."""
     H = info.numharm
     S = info.nsteps
     seg = S * sum(4 * info.fftlen[h] for h in range(H))
@@ -45,7 +48,7 @@ def alg_bytes(info, n: int) -> dict:
         "trial": total,
         "corr_sub": S * sum(4 * info.fftlen[h] for h in range(1, H)) + 4 * sub,
         "corr_fund": S * 4 * info.fftlen[0] + 4 * info.cells,
-        "harmsum": 4 * sub + 8 * info.cells,
+        "harmsum": 4 * sub + 4 * info.cells,
     }
 
 

@@ -32,7 +32,8 @@ namespace fdas {
     static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
-// fundamental powers on entry and the summed powers on exit.
+// fundamental powers on entry and, when store is set, the summed powers on exit
+// (every cell is summed and thresholded either way).
 template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
 __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
                                                        const float* __restrict__ sub,
@@ -40,7 +41,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                                                        const int* __restrict__ fullrlo,
                                                        const int* __restrict__ rlo_h,
                                                        fdas_candidate* __restrict__ cands,
-                                                       int* __restrict__ ncand)
+                                                       int* __restrict__ ncand, int store)
 {
     HS_TILE_CONSTANTS
     __shared__ __align__(16) float tile[HS_LDS];
@@ -163,13 +164,15 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
         h = h2;
     }
 
+    if (store) {
 #pragma unroll
-    for (int r = 0; r < HS_RJ; ++r)
+        for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
-        for (int k = 0; k < HS_CPT; ++k) {
-            const int col = c0 + t + k * HS_T;
-            if (r < nrow && col < c0 + ncol) base_row[(size_t)r * U + col] = acc[r][k];
-        }
+            for (int k = 0; k < HS_CPT; ++k) {
+                const int col = c0 + t + k * HS_T;
+                if (r < nrow && col < c0 + ncol) base_row[(size_t)r * U + col] = acc[r][k];
+            }
+    }
 
     // rare path: stage sums of the marked cells, re-accumulated in the same order
     if (hit) {
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
 template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
 static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0,
                      int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
-                     fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
+                     fdas_candidate* d_cands, int* d_ncand, int store, hipStream_t s)
 {
     for (int h = 1; h < hs.H; ++h) {
         if ((hs.W[h] & 3) || (hs.sub_off[h] & 3) || (hs.sub_stride[h] & 3)) {
@@ -218,7 +221,7 @@ static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, 
     dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
               (unsigned)nsteps);
     hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS>), grid, dim3(HS_T), 0, s,
-                       d_plane, d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand);
+                       d_plane, d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, store);
     return check_launch("harmsum_kernel");
 }
 
@@ -257,10 +260,11 @@ void harmsum_variant_shape(int v, int* threads, int* rows, int* cols)
 
 int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
                    int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
-                   int* d_ncand, hipStream_t s)
+                   int* d_ncand, bool store_plane, hipStream_t s)
 {
     if (nsteps <= 0) return FDAS_OK;
-#define HS_ARGS d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, s
+    const int st = store_plane ? 1 : 0;
+#define HS_ARGS d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, st, s
     switch (g_hs_variant.load()) {
         case 1: return harmsum_t<256, 4, 1024, 8192>(HS_ARGS);
         case 2: return harmsum_t<256, 2, 2048, 16384>(HS_ARGS);

@@ -605,10 +605,12 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         }
         // The harmonic sum as its own tiled kernel: summing inside the fundamental
         // correlation's epilogue (gathering sub-harmonic cells after each inverse
-        // FFT) measured 1.3-1.5x slower at FFT lengths 4096 and 8192.
+        // FFT) measured 1.3-1.5x slower at FFT lengths 4096 and 8192.  The summed
+        // plane is stored only into a caller's buffer: the internal plane's sole
+        // consumer is the threshold, which the kernel applies to every cell.
         TimedLaunch tl(pl, FDAS_K_HARMSUM);
         FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo, pl->d_rlo_h,
-                                pl->d_cands, pl->d_ncand, pl->stream));
+                                pl->d_cands, pl->d_ncand, d_planes != nullptr, pl->stream));
     }
     return FDAS_OK;
 }

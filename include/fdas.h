@@ -219,7 +219,8 @@ int fdas_prepare(fdas_plan *plan, const float *d_series);
  * NULL the summed f-fdot plane of step s is written to
  * d_planes + (s-step0)*numz*uselen (row z, column i).  Candidates append. */
 int fdas_run_steps(fdas_plan *plan, int step0, int nsteps, float *d_planes);
-/* prepare + all steps (one DM trial). */
+/* prepare + all steps (one DM trial).  Every cell is harmonic-summed and
+ * thresholded; the summed planes are not kept (use fdas_run_steps for them). */
 int fdas_search(fdas_plan *plan, const float *d_series);
 /* Copy of the whitened spectrum (nbins complex) to device memory d_out. */
 int fdas_get_spectrum(fdas_plan *plan, fdas_complex *d_out);
