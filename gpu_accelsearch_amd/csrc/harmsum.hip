@@ -81,31 +81,61 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     };
     if (hs.H <= 1) mark_fundamental();
 
-    int stage = 0;
-    for (int h = 1; h < hs.H;) {
-        // harmonics [h, h2) whose rectangles fit in LDS together; a rectangle row
-        // holds columns [ri0 & ~3, ...) rounded up to a multiple of 4
-        int h2 = h, used = 0;
-        while (h2 < hs.H) {
-            const int c = hs.c[h2], lm = hs.lm[h2], r2 = 2 * rl[h2];
-            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[h2]) -
-                           hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[h2]) + 1;
-            const int ri0 = hs_rind(q0 + c0, c, lm, r2);
-            const int nca = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - (ri0 & ~3) + 3) & ~3;
-            if (used + nr * nca > HS_LDS) break;
-            used += nr * nca;
-            ++h2;
+    // Geometry of every harmonic's rectangle, computed once per tile by one
+    // thread per harmonic and read back by all waves as uniform values:
+    // [0] first staged column (4-aligned), [1] staged row width (multiple of 4),
+    // [2] rows, [3] c, [4] log2 m, [5] 2 rlo_h, [6 + r] row of fundamental row r
+    // relative to the first, [6 + HS_RJ] LDS offset of the rectangle in its
+    // round; gbase[h]: element offset of the rectangle's first staged value.
+    constexpr int GEO = 7 + HS_RJ;
+    __shared__ int geo[MAX_HARM][GEO];
+    __shared__ long long gbase[MAX_HARM];
+    __shared__ int rounds[MAX_HARM + 1];
+    __shared__ int nrounds;
+    if (t > 0 && t < hs.H) {
+        const int c = hs.c[t], lm = hs.lm[t], r2 = 2 * rl[t];
+        const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[t]);
+        const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
+        geo[t][0] = ri0a;
+        geo[t][1] = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) & ~3;
+        geo[t][2] = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[t]) - zi0 + 1;
+        geo[t][3] = c;
+        geo[t][4] = lm;
+        geo[t][5] = r2;
+#pragma unroll
+        for (int r = 0; r < HS_RJ; ++r)
+            geo[t][6 + r] = hs_zind(hs.zlo0 + DZ * (j0 + min(r, nrow - 1)), c, lm, hs.zlo[t]) - zi0;
+        gbase[t] = hs.sub_off[t] + (long long)s * hs.sub_stride[t] + (long long)zi0 * hs.W[t] + ri0a;
+    }
+    __syncthreads();
+    if (t == 0) {
+        // harmonics [rounds[i], rounds[i+1]) whose rectangles fit in LDS together
+        int n = 0, used = 0;
+        rounds[0] = 1;
+        for (int h = 1; h < hs.H; ++h) {
+            const int sz = geo[h][1] * geo[h][2];
+            if (used + sz > HS_LDS) {
+                rounds[++n] = h;
+                used = 0;
+            }
+            geo[h][6 + HS_RJ] = used;
+            used += sz;
         }
-        __syncthreads();   // earlier readers of the tile are done
-        int base = 0;
+        rounds[++n] = hs.H;
+        nrounds = hs.H > 1 ? n : 0;
+    }
+    __syncthreads();
+    auto u = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
+
+    int stage = 0;
+    const int nrnd = u(nrounds);
+    for (int rd = 0; rd < nrnd; ++rd) {
+        const int h = u(rounds[rd]), h2 = u(rounds[rd + 1]);
+        if (rd > 0) __syncthreads();   // earlier readers of the tile are done
         for (int g = h; g < h2; ++g) {
-            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
-            const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
-            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
-            const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
-            const int nq = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) >> 2;
-            const float4* src = reinterpret_cast<const float4*>(
-                sub + hs.sub_off[g] + s * hs.sub_stride[g] + (long long)zi0 * hs.W[g] + ri0a);
+            const int nca = u(geo[g][1]), nr = u(geo[g][2]), base = u(geo[g][6 + HS_RJ]);
+            const int nq = nca >> 2;
+            const float4* src = reinterpret_cast<const float4*>(sub + gbase[g]);
             const int wq = hs.W[g] >> 2;
             float4 v[HS_MAXR][HS_MAXQ];
 #pragma unroll
@@ -127,16 +157,11 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                     const int q = t + k * HS_T;
                     if (rr < nr && q < nq) dst[rr * nq + q] = v[rr][k];
                 }
-            base += nr * nq * 4;
         }
         __syncthreads();
-        base = 0;
         for (int g = h; g < h2; ++g) {
-            const int c = hs.c[g], lm = hs.lm[g], r2 = 2 * rl[g];
-            const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[g]);
-            const int nr = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[g]) - zi0 + 1;
-            const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
-            const int nca = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) & ~3;
+            const int ri0a = u(geo[g][0]), nca = u(geo[g][1]), base = u(geo[g][6 + HS_RJ]);
+            const int c = u(geo[g][3]), lm = u(geo[g][4]), r2 = u(geo[g][5]);
             int ri[HS_CPT];
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
@@ -145,11 +170,10 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
             }
 #pragma unroll
             for (int r = 0; r < HS_RJ; ++r) {
-                const int zi = hs_zind(hs.zlo0 + DZ * (j0 + min(r, nrow - 1)), c, lm, hs.zlo[g]) - zi0;
+                const int zi = u(geo[g][6 + r]);
 #pragma unroll
                 for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[base + zi * nca + ri[k]];
             }
-            base += nr * nca;
             if (((g + 1) & g) == 0) {
                 ++stage;
                 const float pc = hs.powcut[stage];
@@ -161,7 +185,6 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                             hit |= 1u << (r * HS_CPT + k);
             }
         }
-        h = h2;
     }
 
     if (store) {
