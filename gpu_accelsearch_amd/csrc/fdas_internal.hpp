@@ -146,7 +146,7 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s);
 // Fused correlation + inverse FFT + power for one harmonic over a batch of steps:
 // out[(s*Z + z)*W + i] = |IFFT(A_s * conj(K_z))[off + i]|^2 / L^2, i < W, with
 // A_s at seg + s*seg_stride + seg_off.  Reduced precision: bank/seg packed, with
-// exponents bank_exp[z] and seg_exp[s*H + h]; out is FP16 when out_half.
+// exponents bank_exp[z] and seg_exp[s*H + h].
 struct CorrArgs {
     const void* bank;
     const int* bank_exp;
@@ -162,7 +162,6 @@ struct CorrArgs {
     int off;
     int W;
     void* out;
-    int out_half;
     int prec;
 };
 // variant < 0: the shape currently selected for length L (autotuner table).

@@ -195,7 +195,7 @@ namespace {
 // Brackets one launch with events when timing is on.
 struct TimedLaunch {
     fdas_plan* pl;
-    hipStream_t st;
+    hipStream_t st;   // the stream the bracketed launch goes to
     size_t idx = (size_t)-1;
     TimedLaunch(fdas_plan* p, int cat) : pl(p), st(p->stream)
     {
@@ -589,7 +589,6 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
             a.off = pl->off[h];
             a.W = W;
             a.out = out;
-            a.out_half = 0;
             a.prec = pl->prec;
             return a;
         };
@@ -908,11 +907,12 @@ extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capac
         FDAS_HIP(hipStreamSynchronize(pl->stream));
     }
     // kernels record the item; report (trial, step within the trial)
+    std::vector<int> first(pl->trials.size());
+    for (size_t t = 0; t < first.size(); ++t) first[t] = pl->trials[t].item0;
     for (auto& c : all) {
-        int t = (int)pl->trials.size() - 1;
-        while (t > 0 && pl->trials[t].item0 > c.step) --t;
+        const int t = (int)(std::upper_bound(first.begin(), first.end(), c.step) - first.begin()) - 1;
         c.trial = t;
-        c.step -= pl->trials[t].item0;
+        c.step -= first[t];
     }
     std::sort(all.begin(), all.end(), [](const fdas_candidate& a, const fdas_candidate& b) {
         if (a.trial != b.trial) return a.trial < b.trial;

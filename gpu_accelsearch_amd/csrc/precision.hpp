@@ -5,9 +5,8 @@
 // round to nearest even) together with one power-of-two exponent per template row
 // or per segment, chosen so that the largest component maps below 2^15.  Scaling
 // by a power of two is exact, so the exponents are folded into the power scale
-// (|y|^2 scales with 2^(2 (e_template + e_segment))).  All FFT arithmetic and all
-// power accumulation stay in FP32.  PStore<unsigned short> (FP16 power planes,
-// saturating at 65504) is available to the correlation and harmonic-sum kernels.
+// (|y|^2 scales with 2^(2 (e_template + e_segment))).  All FFT arithmetic, power
+// planes and power accumulation stay in FP32.
 #pragma once
 
 #include <hip/hip_fp16.h>
@@ -64,18 +63,13 @@ struct CStore<PREC_BF16> {
     }
 };
 
-// Power-plane element: float, or FP16 (saturating at the largest finite value).
+// Power-plane element (FP32 in every mode).
 template <typename E>
 struct PStore;
 template <>
 struct PStore<float> {
     __device__ __forceinline__ static float get(float v) { return v; }
     __device__ __forceinline__ static float put(float v) { return v; }
-};
-template <>
-struct PStore<unsigned short> {
-    __device__ __forceinline__ static float get(unsigned short v) { return f16_to_f32(v); }
-    __device__ __forceinline__ static unsigned short put(float v) { return f32_to_f16(fminf(v, 65504.f)); }
 };
 
 // Exponent e with max_abs * 2^-e < 2^15 (0 for an all-zero block).

@@ -512,9 +512,7 @@ template <int L, int T, int G, int SPLIT>
 static int corr_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
-        case PREC_F16:
-            return a.out_half ? corr_t<L, T, G, SPLIT, PREC_F16, unsigned short>(a, s)
-                              : corr_t<L, T, G, SPLIT, PREC_F16, float>(a, s);
+        case PREC_F16: return corr_t<L, T, G, SPLIT, PREC_F16, float>(a, s);
         case PREC_BF16: return corr_t<L, T, G, SPLIT, PREC_BF16, float>(a, s);
         default: return corr_t<L, T, G, SPLIT, PREC_F32, float>(a, s);
     }
@@ -541,10 +539,6 @@ static int corr_l(const CorrArgs& a, hipStream_t s, int v)
 int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
 {
     if (a.nsteps <= 0) return FDAS_OK;
-    if (a.out_half && a.prec != PREC_F16) {
-        set_error("FP16 power planes need FP16 inputs");
-        return FDAS_ERR_INVALID;
-    }
 
     const int v = variant >= 0 ? variant : corr_variant(L);
     switch (L) {

@@ -144,7 +144,6 @@ int time_corr_variants(int L, int prec, hipStream_t s, std::vector<double>* us)
     ca.off = off;
     ca.W = W;
     ca.out = out;
-    ca.out_half = 0;
     ca.prec = prec;
     const int nv = corr_variant_count(L);
     us->assign(nv, 0.0);
