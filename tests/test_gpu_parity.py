@@ -239,3 +239,26 @@ def test_search_is_deterministic(fd):
         np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
         c = s.run_steps(120, 5)
         np.testing.assert_array_equal(c.view(np.uint32), a[20:25].view(np.uint32))
+
+
+def test_plan_zmax_zero(fd, oracle):
+    """zmax 0: one template (the z = 0 interpolation kernel), harmonics 1 and 4."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 19
+    x = synth.plain_series(n, seed=14, r0=21000.3, z0=0.0, power=90.0)
+    for numharm, fftlen in ((1, 1024), (4, 2048)):
+        print_case("plan zmax 0, numharm", numharm)
+        _plan_vs_oracle(fd, oracle, x, n, 0, numharm, fftlen, [0, 100, 250])
+
+
+def test_plan_largest_shape(fd, oracle):
+    """The largest serving shape: 2^25 samples, zmax 600 (601 templates),
+    16 harmonics, fftlen 16384 (correlation lengths up to 16384)."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 25
+    rng = np.random.default_rng(15)
+    x = rng.standard_normal(n).astype(np.float32)
+    x = synth.add_chirp(x, 400000.3, 250.0, 200.0)
+    st = synth.steps_covering(400000.3, 2147, 14942)
+    print_case("plan 2^25 zmax 600 numharm 16 steps", [0, st])
+    _plan_vs_oracle(fd, oracle, x, n, 600, 16, 16384, [0, st])
