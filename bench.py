@@ -223,7 +223,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=96, help="oracle steps sampled")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16"],
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "i16"],
                     help="storage of spectra segments and templates (arithmetic is FP32)")
     ap.add_argument("--batch-trials", type=int, default=8,
                     help="DM trials of the batched reduced-precision form (0: skip)")
@@ -338,6 +338,7 @@ def main() -> int:
         bt = fd.autotune(rp, fftlens=(4096, 8192, 16384))
         out["batch_bf16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "bf16", bt["fftlen"])
         out["batch_bf16"]["fftlen_ms_per_trial"] = bt["ms_per_trial"]
+        out["batch_i16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "i16", bt["fftlen"])
         out["batch_fp32"] = batch_form(series, n, zmax, numharm, args.batch_trials, "fp32", fftlen)
         ref = out.get("stage_form_fp32", {}).get("dm_trials_per_s")
         out["batch_bf16"]["vs_fused_fp32_single"] = out["batch_bf16"]["dm_trials_per_s"] * per_trial

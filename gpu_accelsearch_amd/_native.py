@@ -16,7 +16,7 @@ MAX_STAGES = 5
 FDAS_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
           4: "unsupported shape"}
-PREC = {"fp32": 0, "fp16": 1, "bf16": 2}
+PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "i16": 3}
 
 
 class FdasComplex(C.Structure):

@@ -242,7 +242,7 @@ int validate(const fdas_params& p)
         set_error("fftlen %d must be a power of two in [1024, 16384]", p.fftlen);
         return FDAS_ERR_UNSUPPORTED;
     }
-    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_BF16) {
+    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_I16) {
         set_error("precision mode %d unknown", p.precision);
         return FDAS_ERR_INVALID;
     }

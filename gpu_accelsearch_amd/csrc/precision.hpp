@@ -2,7 +2,8 @@
 //
 // FP32 mode stores complex values as two floats.  The reduced-precision modes
 // store a complex value as two 16-bit halves packed in 32 bits (FP16 or BF16,
-// round to nearest even) together with one power-of-two exponent per template row
+// round to nearest even; or int16 block floating point) together with one
+// power-of-two exponent per template row
 // or per segment, chosen so that the largest component maps below 2^15.  Scaling
 // by a power of two is exact, so the exponents are folded into the power scale
 // (|y|^2 scales with 2^(2 (e_template + e_segment))).  All FFT arithmetic, power
@@ -15,7 +16,7 @@
 
 namespace fdas {
 
-enum Prec : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2 };
+enum Prec : int { PREC_F32 = 0, PREC_F16 = 1, PREC_BF16 = 2, PREC_I16 = 3 };
 
 __device__ __forceinline__ unsigned short f32_to_f16(float x) { return __half_as_ushort(__float2half_rn(x)); }
 __device__ __forceinline__ float f16_to_f32(unsigned short b) { return __half2float(__ushort_as_half(b)); }
@@ -60,6 +61,30 @@ struct CStore<PREC_BF16> {
     __device__ __forceinline__ static T put(cplx v)
     {
         return (unsigned)f32_to_bf16(v.x) | ((unsigned)f32_to_bf16(v.y) << 16);
+    }
+};
+
+// 16-bit block floating point: two int16 with the row's / segment's exponent.
+// The exponent puts the block's largest component in [2^14, 2^15), so values
+// near the top of the block keep 15 significant bits (rounded to nearest,
+// clamped to +-32767); smaller ones keep fewer.
+__device__ __forceinline__ unsigned short f32_to_i16(float x)
+{
+    const float r = fminf(fmaxf(rintf(x), -32767.f), 32767.f);
+    return (unsigned short)(short)r;
+}
+__device__ __forceinline__ float i16_to_f32(unsigned short b) { return (float)(short)b; }
+
+template <>
+struct CStore<PREC_I16> {
+    using T = unsigned;
+    __device__ __forceinline__ static cplx get(T v)
+    {
+        return mk(i16_to_f32((unsigned short)(v & 0xffffu)), i16_to_f32((unsigned short)(v >> 16)));
+    }
+    __device__ __forceinline__ static T put(cplx v)
+    {
+        return (unsigned)f32_to_i16(v.x) | ((unsigned)f32_to_i16(v.y) << 16);
     }
 };
 

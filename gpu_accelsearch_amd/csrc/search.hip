@@ -371,6 +371,7 @@ static int seg_prep_p(const SegPrepArgs& a, hipStream_t s)
     switch (a.prec) {
         case PREC_F16: return seg_prep_t<NH, PREC_F16>(a, s);
         case PREC_BF16: return seg_prep_t<NH, PREC_BF16>(a, s);
+        case PREC_I16: return seg_prep_t<NH, PREC_I16>(a, s);
         default: return seg_prep_t<NH, PREC_F32>(a, s);
     }
 }
@@ -514,6 +515,7 @@ static int corr_p(const CorrArgs& a, hipStream_t s)
     switch (a.prec) {
         case PREC_F16: return corr_t<L, T, G, SPLIT, PREC_F16, float>(a, s);
         case PREC_BF16: return corr_t<L, T, G, SPLIT, PREC_BF16, float>(a, s);
+        case PREC_I16: return corr_t<L, T, G, SPLIT, PREC_I16, float>(a, s);
         default: return corr_t<L, T, G, SPLIT, PREC_F32, float>(a, s);
     }
 }
@@ -563,6 +565,9 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
                            static_cast<unsigned*>(d_out), d_row_exp);
     else if (prec == PREC_BF16)
         hipLaunchKernelGGL(bank_pack_kernel<PREC_BF16>, dim3(Z), dim3(256), 0, s, d_bank, L,
+                           static_cast<unsigned*>(d_out), d_row_exp);
+    else if (prec == PREC_I16)
+        hipLaunchKernelGGL(bank_pack_kernel<PREC_I16>, dim3(Z), dim3(256), 0, s, d_bank, L,
                            static_cast<unsigned*>(d_out), d_row_exp);
     else {
         set_error("bank packing needs a reduced-precision mode");

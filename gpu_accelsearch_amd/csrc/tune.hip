@@ -95,6 +95,10 @@ int fill_store(void* x, long n, int prec, unsigned seed, hipStream_t s)
             hipLaunchKernelGGL(fill_store_kernel<PREC_BF16>, dim3(blocks(n)), dim3(256), 0, s,
                                static_cast<unsigned*>(x), n, seed);
             break;
+        case PREC_I16:
+            hipLaunchKernelGGL(fill_store_kernel<PREC_I16>, dim3(blocks(n)), dim3(256), 0, s,
+                               static_cast<unsigned*>(x), n, seed);
+            break;
         default:
             hipLaunchKernelGGL(fill_store_kernel<PREC_F32>, dim3(blocks(n)), dim3(256), 0, s,
                                static_cast<cplx*>(x), n, seed);

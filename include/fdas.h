@@ -142,9 +142,10 @@ enum {
 };
 
 /* Storage of spectra segments and templates (FFT arithmetic, power planes and
- * power accumulation are FP32 in every mode): FP32, FP16 or BF16.  Reduced-
- * precision values carry power-of-two exponents per template row and segment. */
-enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2 };
+ * power accumulation are FP32 in every mode): FP32, FP16, BF16, or I16 (int16
+ * pairs: 16-bit block floating point).  Reduced-precision values carry power-of-
+ * two exponents per template row and segment. */
+enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2, FDAS_PREC_I16 = 3 };
 
 #define FDAS_MAX_HARM 16
 #define FDAS_MAX_STAGES 5

@@ -1,5 +1,5 @@
-"""Reduced-precision storage (FP16 / BF16 spectra segments and templates, FP16
-sub-harmonic power planes in FP16 mode) against the FP64 oracle.
+"""Reduced-precision storage of spectra segments and templates (FP16, BF16, and
+int16 block floating point) against the FP64 oracle; power planes stay FP32.
 
 Reported per mode: the largest |dP| / max(P_ref, 1) over the checked steps, and
 agreement of the candidate sets wherever the oracle's stage sums are farther
@@ -12,7 +12,7 @@ from conftest import print_case
 pytestmark = pytest.mark.gpu
 
 # measured bounds (precision section)
-TOL = {"fp16": 2e-3, "bf16": 2e-2}
+TOL = {"fp16": 2e-3, "bf16": 2e-2, "i16": 1e-3}
 
 
 def power_err(p, pref):
@@ -26,7 +26,7 @@ def fd():
     return fd
 
 
-@pytest.mark.parametrize("mode", ["fp16", "bf16"])
+@pytest.mark.parametrize("mode", ["fp16", "bf16", "i16"])
 def test_precision_plain(fd, oracle, mode):
     from gpu_accelsearch_amd import synth
     n = 1 << 20
@@ -44,7 +44,7 @@ def test_precision_plain(fd, oracle, mode):
         ref.close()
 
 
-@pytest.mark.parametrize("mode", ["fp16", "bf16"])
+@pytest.mark.parametrize("mode", ["fp16", "bf16", "i16"])
 def test_precision_production_candidates(fd, oracle, mode):
     """2^23 samples, zmax 200, 8 harmonics: powers on the pulsar steps and the
     candidate sets there."""
@@ -88,7 +88,7 @@ def test_precision_production_candidates(fd, oracle, mode):
 C4_PULSARS = ((120000.3, 30.0, 21.0), (250000.6, -25.0, 45.0), (410000.2, 12.0, 90.0),
               (600000.8, -35.0, 200.0), (900000.4, 20.0, 455.0))
 C4_POWCUT = (30.0, 36.0, 45.0, 60.0, 0.0)
-C4_TOL = {"fp32": 1e-4, "fp16": 2e-3, "bf16": 2e-2}
+C4_TOL = {"fp32": 1e-4, "fp16": 2e-3, "bf16": 2e-2, "i16": 1e-3}
 
 
 @pytest.fixture(scope="module")
@@ -103,7 +103,7 @@ def c4(oracle):
     return x, n, steps, planes
 
 
-@pytest.mark.parametrize("mode", ["fp32", "fp16", "bf16"])
+@pytest.mark.parametrize("mode", ["fp32", "fp16", "bf16", "i16"])
 def test_precision_workload(fd, c4, mode):
     """Max relative power error and candidate agreement per storage mode."""
     x, n, steps, planes = c4
