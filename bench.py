@@ -325,8 +325,10 @@ def main() -> int:
         "kernels": kern,
         "candidates": int(len(cands)),
         "autotune": None if tune is None else
-        {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"], "split": v["split"]}
-         for k, v in tune["corr"].items()},
+        {"corr": {str(k): {"threads": v["threads"], "steps_per_wg": v["steps_per_wg"], "split": v["split"]}
+                  for k, v in tune["corr"].items()},
+         "harmsum": {k: tune["harmsum"][k] for k in ("threads", "rows", "cols")}},
+        "fft_lengths": [h[2] for h in s.harmonics],
     }
     spec = s.spectrum() if args.stage_steps > 0 else None
     s.close()

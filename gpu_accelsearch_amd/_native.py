@@ -17,6 +17,7 @@ FDAS_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
           4: "unsupported shape"}
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "i16": 3}
+LENGTHS = {"shortest": 0, "reference": 1}
 
 
 class FdasComplex(C.Structure):
@@ -41,7 +42,7 @@ class FdasParams(C.Structure):
     _fields_ = [("n", C.c_long), ("dt", C.c_float), ("zmax", C.c_int), ("numharm", C.c_int),
                 ("fftlen", C.c_int), ("uselen", C.c_int), ("precision", C.c_int),
                 ("chunk_steps", C.c_int), ("max_candidates", C.c_int),
-                ("powcut", C.c_float * MAX_STAGES)]
+                ("powcut", C.c_float * MAX_STAGES), ("fft_lengths", C.c_int)]
 
 
 class FdasPlanInfo(C.Structure):
@@ -117,7 +118,7 @@ class FdasInf(C.Structure):
                 ("barycentred", C.c_int)]
 
 
-TUNE_MAX = 8
+TUNE_MAX = 12
 TUNE_MAXV = 8
 
 

@@ -82,6 +82,7 @@ class SearchParams:
     chunk_steps: int = 0
     max_candidates: int = 0
     powcut: tuple = field(default_factory=lambda: (0.0,) * N.MAX_STAGES)
+    fft_lengths: str = "shortest"   # sub-harmonic FFT lengths: "shortest" or "reference"
 
     def to_c(self) -> N.FdasParams:
         p = N.FdasParams()
@@ -91,6 +92,7 @@ class SearchParams:
         pc = list(self.powcut) + [0.0] * N.MAX_STAGES
         for i in range(N.MAX_STAGES):
             p.powcut[i] = float(pc[i])
+        p.fft_lengths = N.LENGTHS[self.fft_lengths]
         return p
 
 

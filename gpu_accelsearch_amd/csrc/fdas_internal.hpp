@@ -38,6 +38,12 @@ bool fft_row_supported(long n);
 const cplx* fft_twiddles(int n, int* err);
 // Same for a transform run by t threads (the pass radices depend on n/t).
 const cplx* fft_twiddles_t(int n, int t, int* err);
+// Radix-3 step table of the length-3m transforms: w^k, w^(2k), w = exp(2 pi i/3m),
+// k in [0, m) (cached per device).
+const cplx* fft_r3_twiddles(int m, int* err);
+// Rows of length 3m (m = 256..2048) forward-transformed into the decimated order:
+// out[row*3m + r*m + j] = DFT_-1(in[row*3m + .])[3j + r].  In place allowed.
+int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s);
 
 // Row transforms: out[row*out_stride + k] = scale * DFT_dir(in[row*in_stride + .])[k]
 // (times exp(dir 2 pi i (row*k mod tw_m)/tw_m) when tw_m > 0).  In place allowed.
@@ -97,6 +103,8 @@ long long h_next_pow2(long long x);
 int h_fast_log2(int n);
 int h_fft_length(int uselen, int numharm, int harmnum, int zmax);
 int h_max_uselen(int fftlen, int zmax);
+// FFT length of harmonic cur/mx under FDAS_LENGTHS_* mode (see include/fdas.h).
+int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode);
 void h_harmonic_order(int numharm, int* cur, int* max);
 
 // Fill bank[k*L .. (k+1)*L) with the FFT'd template rows z_k = -zmax_h + 2k,

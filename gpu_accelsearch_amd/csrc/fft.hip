@@ -59,6 +59,9 @@ static std::vector<cplx> host_twiddles_for(int n, int t)
         // the other shapes the correlation and segment kernels instantiate
         if (n == 128 && t == 64) return host_twiddles<128, 64>();
         if (n == 1024 && t == 128) return host_twiddles<1024, 128>();
+        if (n == 1024 && t == 256) return host_twiddles<1024, 256>();
+        if (n == 512 && t == 128) return host_twiddles<512, 128>();
+        if (n == 2048 && t == 512) return host_twiddles<2048, 512>();
         if (n == 2048 && t == 256) return host_twiddles<2048, 256>();
         if (n == 4096 && t == 512) return host_twiddles<4096, 512>();
         if (n == 8192 && t == 512) return host_twiddles<8192, 512>();
@@ -114,7 +117,99 @@ const cplx* fft_twiddles_t(int n, int t, int* err)
     return d;
 }
 
+const cplx* fft_r3_twiddles(int m, int* err)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, cplx*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *err = FDAS_ERR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, m);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    if (m < 64 || m > 4096 || (m & (m - 1))) {
+        set_error("no length-3m transform with m = %d", m);
+        *err = FDAS_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    std::vector<cplx> h(2 * (size_t)m);
+    build_radix3_twiddles(m, h.data());
+    cplx* d = nullptr;
+    hipError_t e = hipMalloc(&d, h.size() * sizeof(cplx));
+    if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *err = hip_fail(e, "radix-3 twiddle table", __FILE__, __LINE__);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
 // ------------------------------------------------------------------ kernels
+// Forward transform of rows of length 3M into the decimated order (template
+// banks of the length-3M correlations).  Every thread's loads precede the first
+// barrier of the transform and its stores follow the last, so in place is safe.
+template <int M>
+__global__ __launch_bounds__(FftCfg<M>::T) void fft_r3_rows_kernel(const cplx* in, cplx* out,
+                                                                    const cplx* __restrict__ tw,
+                                                                    const cplx* __restrict__ tw3)
+{
+    constexpr int T = FftCfg<M>::T;
+    using F = BlockFFT<M, T, -1>;
+    constexpr int E = F::E;
+    static_assert(F::NPASS >= 2, "in-place rows need a barrier between loads and stores");
+    __shared__ cplx lds[3 * F::LDS_ELEMS];
+    const int t = threadIdx.x;
+    const cplx* x = in + (size_t)blockIdx.x * 3 * M;
+    cplx* y = out + (size_t)blockIdx.x * 3 * M;
+    cplx v[3][E];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[a][i] = x[F::in_index(t, i) + M * a];
+    Radix3<M, T>::fwd_split(v, t, tw3);
+    F::template run_multi<3>(v, lds, tw, t);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int i = 0; i < E; ++i) y[M * a + F::out_index(t, i)] = v[a][i];
+}
+
+template <int M>
+static int fft_r3_rows_t(const cplx* in, cplx* out, long nrows, hipStream_t s)
+{
+    int err = FDAS_OK;
+    const cplx* tw = fft_twiddles(M, &err);
+    if (!tw) return err;
+    const cplx* tw3 = fft_r3_twiddles(M, &err);
+    if (!tw3) return err;
+    hipLaunchKernelGGL(fft_r3_rows_kernel<M>, dim3((unsigned)nrows), dim3(FftCfg<M>::T), 0, s, in,
+                       out, tw, tw3);
+    return check_launch("fft_r3_rows_kernel");
+}
+
+int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s)
+{
+    if (nrows <= 0) return FDAS_OK;
+    if (nrows > 2147483647L) {
+        set_error("fft_r3_rows: %ld rows exceed the grid limit", nrows);
+        return FDAS_ERR_INVALID;
+    }
+    switch (m) {
+        case 256: return fft_r3_rows_t<256>(in, out, nrows, s);
+        case 512: return fft_r3_rows_t<512>(in, out, nrows, s);
+        case 1024: return fft_r3_rows_t<1024>(in, out, nrows, s);
+        case 2048: return fft_r3_rows_t<2048>(in, out, nrows, s);
+        default:
+            set_error("no length-3m row transform with m = %d", m);
+            return FDAS_ERR_UNSUPPORTED;
+    }
+}
+
 template <int N, int DIR>
 __global__ __launch_bounds__(FftCfg<N>::T) void fft_rows_kernel(
     const cplx* __restrict__ in, cplx* __restrict__ out, long in_stride, long out_stride,

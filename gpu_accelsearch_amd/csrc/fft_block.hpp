@@ -271,6 +271,97 @@ struct BlockFFT {
         compute<0>(v, t, tw);
         passes_from<1>(v, lds, tw, t);
     }
+
+    // NA independent transforms in lock step, sharing each exchange's barriers;
+    // transform a uses lds[a * LDS_ELEMS, (a + 1) * LDS_ELEMS).
+    template <int NA, int P>
+    FDAS_HD static void passes_multi(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    {
+        if constexpr (P < NPASS) {
+            block_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) store<P - 1>(v[a], lds + a * LDS_ELEMS, t);
+            block_sync();
+#pragma unroll
+            for (int a = 0; a < NA; ++a) load<P>(v[a], lds + a * LDS_ELEMS, t);
+#pragma unroll
+            for (int a = 0; a < NA; ++a) compute<P>(v[a], t, tw);
+            passes_multi<NA, P + 1>(v, lds, tw, t);
+        }
+    }
+    template <int NA>
+    FDAS_HD static void run_multi(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) compute<0>(v[a], t, tw);
+        passes_multi<NA, 1>(v, lds, tw, t);
+    }
+};
+
+// ------------------------------------------------------------------ length 3M
+// A transform of length N = 3M is three length-M workgroup transforms plus one
+// radix-3 step that each thread applies to its own registers (no extra LDS
+// exchange).  With w = exp(2 pi i / N):
+//   forward:  x[n1 + M n2] (n1 = t + i T held in x[n2][i]) ->
+//             b_r[n1] = w^(-r n1) DFT3(x[n1], x[n1 + M], x[n1 + 2M])[r],
+//             FFT_M(b_r)[m] = X[3m + r]  ("decimated" order, r-major);
+//   inverse:  V_r[n] = V[3n + r];  Y_r = IFFT_M(V_r);
+//             y[k + M q] = sum_r exp(2 pi i r q / 3) w^(r k) Y_r[k].
+// tw3[2k] = w^k and tw3[2k + 1] = w^(2k) for k in [0, M), rounded from double.
+
+// DFT of three values in place, sign DIR: a, b, c <- y0, y1, y2.
+template <int DIR>
+FDAS_HD void dft3(cplx& a, cplx& b, cplx& c)
+{
+    constexpr float h = 8.660254038e-01f;   // sqrt(3) / 2
+    const cplx s = b + c, d = b - c;
+    const cplx m = mk(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
+    const cplx u = DIR > 0 ? mk(-h * d.y, h * d.x) : mk(h * d.y, -h * d.x);
+    a = a + s;
+    b = m + u;
+    c = m - u;
+}
+
+inline void build_radix3_twiddles(int m, cplx* tw3)
+{
+    for (int k = 0; k < m; ++k) {
+        const double a = 6.283185307179586476925286766559 * (double)k / (3.0 * (double)m);
+        tw3[2 * k] = mk((float)cos(a), (float)sin(a));
+        tw3[2 * k + 1] = mk((float)cos(2.0 * a), (float)sin(2.0 * a));
+    }
+}
+
+template <int M, int T>
+struct Radix3 {
+    static constexpr int N = 3 * M;
+    static constexpr int E = M / T;
+    using FI = BlockFFT<M, T, 1>;
+
+    // Forward radix-3 step before the three forward length-M transforms.
+    FDAS_HD static void fwd_split(cplx (*x)[E], int t, const cplx* tw3)
+    {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int n1 = FI::in_index(t, i);
+            dft3<-1>(x[0][i], x[1][i], x[2][i]);
+            const cplx w1 = tw3[2 * n1], w2 = tw3[2 * n1 + 1];
+            x[1][i] = cmul(x[1][i], mk(w1.x, -w1.y));
+            x[2][i] = cmul(x[2][i], mk(w2.x, -w2.y));
+        }
+    }
+    // Inverse radix-3 step after the three inverse length-M transforms: y[q][i]
+    // becomes output out_index(t, i) + M q.
+    FDAS_HD static void inv_combine(cplx (*y)[E], int t, const cplx* tw3)
+    {
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int k = FI::out_index(t, i);
+            y[1][i] = cmul(y[1][i], tw3[2 * k]);
+            y[2][i] = cmul(y[2][i], tw3[2 * k + 1]);
+            dft3<1>(y[0][i], y[1][i], y[2][i]);
+        }
+    }
+    static void build_twiddles(cplx* tw3) { build_radix3_twiddles(M, tw3); }
 };
 
 

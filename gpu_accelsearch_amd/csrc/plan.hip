@@ -74,12 +74,14 @@ using BankKey = std::tuple<int, int, int, int, int, int>;   // device, zmax, num
 static std::mutex g_bank_mu;
 static std::map<BankKey, std::weak_ptr<BankSet>> g_banks;
 
-static int build_banks(int zmax, int H, const int* cur, const int* mx, int uselen, int prec,
-                       hipStream_t s, std::shared_ptr<BankSet>* out)
+// Banks of harmonic h have length Ls[h] (a power of two, or 3*2^k stored in the
+// decimated order); the lengths follow from (zmax, H, uselen, mode).
+static int build_banks(int zmax, int H, const int* cur, const int* mx, const int* Ls, int uselen,
+                       int prec, int mode, hipStream_t s, std::shared_ptr<BankSet>* out)
 {
     int dev = 0;
     FDAS_HIP(hipGetDevice(&dev));
-    const BankKey key{dev, zmax, H, uselen, prec, 0};
+    const BankKey key{dev, zmax, H, uselen, prec, mode};
     std::lock_guard<std::mutex> lk(g_bank_mu);
     auto it = g_banks.find(key);
     if (it != g_banks.end()) {
@@ -93,12 +95,17 @@ static int build_banks(int zmax, int H, const int* cur, const int* mx, int usele
     for (int h = 0; h < H; ++h) {
         const double f = (double)cur[h] / (double)mx[h];
         const int zmax_h = h_calc_required_z(f, zmax);
-        const int Z = (zmax_h / DZ) * 2 + 1, L = h_fft_length(uselen, mx[h], cur[h], zmax);
+        const int Z = (zmax_h / DZ) * 2 + 1, L = Ls[h];
         bs->Z[h] = Z;
         bs->L[h] = L;
         cplx *bank = nullptr, *split = nullptr;
         FDAS_HIP(hipMalloc(&bank, (size_t)Z * L * sizeof(cplx)));
         FDAS_TRY(build_template_bank(bank, zmax_h, Z, L, s));
+        if (L % 3 == 0) {   // no split form at lengths 3*2^k
+            const int r1 = store_bank(bank, Z, L, prec, s, &bs->bankq[h], &bs->bank_exp[h], &bs->bytes);
+            FDAS_TRY(r1);
+            continue;
+        }
         if (hipMalloc(&split, (size_t)Z * L * sizeof(cplx)) != hipSuccess) {
             (void)hipFree(bank);
             set_error("template bank allocation failed");
@@ -246,6 +253,10 @@ int validate(const fdas_params& p)
         set_error("precision mode %d unknown", p.precision);
         return FDAS_ERR_INVALID;
     }
+    if (p.fft_lengths != FDAS_LENGTHS_SHORTEST && p.fft_lengths != FDAS_LENGTHS_REFERENCE) {
+        set_error("FFT length mode %d unknown", p.fft_lengths);
+        return FDAS_ERR_INVALID;
+    }
     return FDAS_OK;
 }
 
@@ -274,7 +285,7 @@ int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
         const double f = (double)pl->cur[h] / (double)pl->mx[h];
         pl->zmax_h[h] = h_calc_required_z(f, p.zmax);
         pl->Z[h] = (pl->zmax_h[h] / DZ) * 2 + 1;
-        pl->L[h] = h_fft_length(U, pl->mx[h], pl->cur[h], p.zmax);
+        pl->L[h] = h_harmonic_length(U, pl->mx[h], pl->cur[h], p.zmax, p.fft_lengths);
         pl->off[h] = NUMBETWEEN * h_z_response_halfwidth((double)-pl->zmax_h[h]);
         pl->zlo[h] = h_calc_required_z(f, -p.zmax);
         pl->seg_off[h] = so;
@@ -390,9 +401,14 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
                 const int lobin = rl - bo;
                 const int hibin = (int)std::ceil(drhi) + bo;
                 const int numdata = hibin - lobin + 1;
-                if (h_next_pow2(numdata) != pl->L[h] / 2) {
-                    set_error("step %d harmonic %d/%d: %d bins to normalise do not round to fftlen/2 = %d",
-                              k, pl->cur[h], pl->mx[h], numdata, pl->L[h] / 2);
+                // power-of-two lengths as the reference; a length 3*2^k must hold
+                // every normalised bin and the template span of the last column
+                const bool fits = pl->L[h] % 3 == 0
+                                      ? numdata <= pl->L[h] / 2 && nr + 2 * pl->off[h] <= pl->L[h]
+                                      : h_next_pow2(numdata) == pl->L[h] / 2;
+                if (!fits) {
+                    set_error("step %d harmonic %d/%d: %d bins to normalise do not fit FFT length %d",
+                              k, pl->cur[h], pl->mx[h], numdata, pl->L[h]);
                     return FDAS_ERR_UNSUPPORTED;
                 }
                 for (size_t c = 0; c < pl->classes.size(); ++c) {
@@ -465,7 +481,8 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
         FDAS_HIP(hipStreamCreate(&pl->stream));
         pl->own_stream = true;
     }
-    FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, U, pl->prec, pl->stream, &pl->banks));
+    FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, pl->L, U, pl->prec, p.fft_lengths, pl->stream,
+                         &pl->banks));
     for (size_t c = 0; c < desc.size(); ++c) {
         SegDesc* dd = nullptr;
         FDAS_TRY(plan_alloc(pl, (void**)&dd, desc[c].size() * sizeof(SegDesc)));

@@ -124,30 +124,45 @@ __device__ __forceinline__ float block_max_abs(const cplx* v, float* red, int t)
     return red[T / 64];
 }
 
-template <int NH, int P>
-__global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
+// Segment transforms of length R*NH (R = 1, or 3 for the lengths 3*2^k): thread
+// counts per power-of-two part.
+template <int NH, int R>
+constexpr int seg_threads()
+{
+    return R == 1 ? FftCfgS<NH>::T : (NH >= 1024 ? 128 : 64);
+}
+
+// Forward transform of one normalised segment of R*NH bins.  Bin n of the
+// segment is held at n = in_index(t, i) + NH*r in v[r][i]; R = 3 adds the
+// radix-3 step, so the output comes in the decimated order A[3m + r] at
+// r*NH + m (what the length-3M correlation reads).
+template <int NH, int R, int P>
+__global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
     const cplx* __restrict__ spec_all, const SegDesc* __restrict__ desc,
     typename CStore<P>::T* __restrict__ seg, long long out_base, int* __restrict__ seg_exp,
-    int eslot_base, const cplx* __restrict__ tw)
+    int eslot_base, const cplx* __restrict__ tw, const cplx* __restrict__ tw3)
 {
-    constexpr int T = FftCfgS<NH>::T;
+    constexpr int T = seg_threads<NH, R>();
     using F = BlockFFT<NH, T, -1>;
-    __shared__ cplx lds[F::LDS_ELEMS];
+    constexpr int E = F::E;
+    __shared__ cplx lds[R * F::LDS_ELEMS];
     __shared__ unsigned hist[256];
     __shared__ int sel[2];
     __shared__ float red[T / 64 + 1];
     const SegDesc d = desc[blockIdx.x];
     const cplx* __restrict__ spec = spec_all + d.spec_off;
     const int t = threadIdx.x;
-    cplx v[F::E];
-    unsigned pb[F::E];
+    cplx v[R][E];
+    unsigned pb[R][E];
 #pragma unroll
-    for (int i = 0; i < F::E; ++i) {
-        const int n = F::in_index(t, i);
-        const long b = (long)d.lobin + n;
-        v[i] = (b >= 0 && b < d.nbins) ? spec[b] : mk(0.f, 0.f);
-        pb[i] = __float_as_uint(power_nc(v[i]));
-    }
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int n = F::in_index(t, i) + NH * r;
+            const long b = (long)d.lobin + n;
+            v[r][i] = (b >= 0 && b < d.nbins) ? spec[b] : mk(0.f, 0.f);
+            pb[r][i] = __float_as_uint(power_nc(v[r][i]));
+        }
     // exact lower median: rank (numdata-1)/2 among the first numdata powers
     unsigned prefix = 0, mask = 0;
     int k = (d.numdata - 1) / 2;
@@ -155,11 +170,13 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
         for (int q = t; q < 256; q += T) hist[q] = 0;
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < F::E; ++i) {
-            const int n = F::in_index(t, i);
-            if (n < d.numdata && (pb[i] & mask) == prefix)
-                atomicAdd(&hist[(pb[i] >> shift) & 255u], 1u);
-        }
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const int n = F::in_index(t, i) + NH * r;
+                if (n < d.numdata && (pb[r][i] & mask) == prefix)
+                    atomicAdd(&hist[(pb[r][i] >> shift) & 255u], 1u);
+            }
         select_digit<T>(hist, sel, k, t);
         prefix |= (unsigned)sel[0] << shift;
         mask |= 255u << shift;
@@ -168,19 +185,27 @@ __global__ __launch_bounds__(FftCfgS<NH>::T) void seg_prep_kernel(
     const float med = __uint_as_float(prefix);
     const float c = quake_rsqrt_s((float)((double)med / kLn2));
 #pragma unroll
-    for (int i = 0; i < F::E; ++i)
-        if (F::in_index(t, i) < d.numdata) v[i] = scale_nc(v[i], c);
-    F::run(v, lds, tw, t);
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            if (F::in_index(t, i) + NH * r < d.numdata) v[r][i] = scale_nc(v[r][i], c);
+    if constexpr (R == 3) Radix3<NH, T>::fwd_split(v, t, tw3);
+    F::template run_multi<R>(v, lds, tw, t);
     typename CStore<P>::T* o = seg + (d.out - out_base);
     if constexpr (P == PREC_F32) {
 #pragma unroll
-        for (int i = 0; i < F::E; ++i) o[F::out_index(t, i)] = v[i];
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < E; ++i) o[NH * r + F::out_index(t, i)] = v[r][i];
     } else {
-        const int e = block_exponent(block_max_abs<T, F::E>(v, red, t));
+        const int e = block_exponent(block_max_abs<T, R * E>(&v[0][0], red, t));
         if (t == 0) seg_exp[d.eslot - eslot_base] = e;
 #pragma unroll
-        for (int i = 0; i < F::E; ++i)
-            o[F::out_index(t, i)] = CStore<P>::put(mk(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e)));
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                o[NH * r + F::out_index(t, i)] =
+                    CStore<P>::put(mk(ldexpf(v[r][i].x, -e), ldexpf(v[r][i].y, -e)));
     }
 }
 
@@ -249,6 +274,63 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
             if (col >= 0 && col < W)
                 o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
         }
+    }
+}
+
+// Length L = 3M form of the same correlation.  The segment comes in the
+// decimated order A'[r][m] = A[3m + r] (m < M/2) and the template row in the
+// decimated order K'[r][n] = K[3n + r], so with X = [A, A] the product at
+// position 3n + r is A'[r][n mod M/2] * conj K'[r][n]: three length-M inverse
+// transforms in lock step, then the in-register radix-3 step gives y[k + Mq].
+template <int M, int G, int T, int P, typename OutE>
+__global__ __launch_bounds__(T) void corr_r3_kernel(
+    const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
+    const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
+    const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
+    OutE* __restrict__ out, const cplx* __restrict__ tw, const cplx* __restrict__ tw3, float scale)
+{
+    using F = BlockFFT<M, T, 1>;
+    using S = typename CStore<P>::T;
+    constexpr int E = F::E, EH = E / 2, MH = M / 2;
+    __shared__ cplx lds[3 * F::LDS_ELEMS];
+    const int z = blockIdx.x, t = threadIdx.x;
+    const int s0 = blockIdx.y * G;
+    const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
+    const S* K = bank + (size_t)z * 3 * M;
+    [[maybe_unused]] S kr[G > 1 ? 3 : 1][G > 1 ? E : 1];
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int i = 0; i < E; ++i) kr[r][i] = K[M * r + F::in_index(t, i)];
+    }
+    const int ek = P == PREC_F32 ? 0 : bank_exp[z];
+    for (int s = s0; s < s1; ++s) {
+        const S* A = seg + s * seg_stride + seg_off;
+        cplx v[3][E];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            cplx a[EH];
+#pragma unroll
+            for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[MH * r + t + i * T]);
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const S kv = G > 1 ? kr[G > 1 ? r : 0][G > 1 ? i : 0] : K[M * r + F::in_index(t, i)];
+                v[r][i] = cmulc(a[i % EH], CStore<P>::get(kv));
+            }
+        }
+        F::template run_multi<3>(v, lds, tw, t);
+        Radix3<M, T>::inv_combine(v, t, tw3);
+        const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
+        OutE* o = out + ((size_t)s * Z + z) * W;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int i = 0; i < E; ++i) {
+                const int col = F::out_index(t, i) + M * q - off;
+                if (col >= 0 && col < W)
+                    o[col] = PStore<OutE>::put((v[q][i].x * v[q][i].x + v[q][i].y * v[q][i].y) * sc);
+            }
     }
 }
 
@@ -353,26 +435,32 @@ __global__ __launch_bounds__(256) void bank_pack_kernel(const cplx* __restrict__
 }
 
 // ------------------------------------------------------------------ launchers
-template <int NH, int P>
+template <int NH, int R, int P>
 static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
 {
+    constexpr int T = seg_threads<NH, R>();
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles_t(NH, FftCfgS<NH>::T, &err);
+    const cplx* tw = fft_twiddles_t(NH, T, &err);
     if (!tw) return err;
-    hipLaunchKernelGGL((seg_prep_kernel<NH, P>), dim3(a.nseg), dim3(FftCfgS<NH>::T), 0, s, a.spec,
-                       a.desc, static_cast<typename CStore<P>::T*>(a.seg), a.out_base,
-                       a.seg_exp, a.eslot_base, tw);
+    const cplx* tw3 = nullptr;
+    if constexpr (R == 3) {
+        tw3 = fft_r3_twiddles(NH, &err);
+        if (!tw3) return err;
+    }
+    hipLaunchKernelGGL((seg_prep_kernel<NH, R, P>), dim3(a.nseg), dim3(T), 0, s, a.spec, a.desc,
+                       static_cast<typename CStore<P>::T*>(a.seg), a.out_base, a.seg_exp,
+                       a.eslot_base, tw, tw3);
     return check_launch("seg_prep_kernel");
 }
 
-template <int NH>
+template <int NH, int R = 1>
 static int seg_prep_p(const SegPrepArgs& a, hipStream_t s)
 {
     switch (a.prec) {
-        case PREC_F16: return seg_prep_t<NH, PREC_F16>(a, s);
-        case PREC_BF16: return seg_prep_t<NH, PREC_BF16>(a, s);
-        case PREC_I16: return seg_prep_t<NH, PREC_I16>(a, s);
-        default: return seg_prep_t<NH, PREC_F32>(a, s);
+        case PREC_F16: return seg_prep_t<NH, R, PREC_F16>(a, s);
+        case PREC_BF16: return seg_prep_t<NH, R, PREC_BF16>(a, s);
+        case PREC_I16: return seg_prep_t<NH, R, PREC_I16>(a, s);
+        default: return seg_prep_t<NH, R, PREC_F32>(a, s);
     }
 }
 
@@ -387,6 +475,10 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
         case 2048: return seg_prep_p<2048>(a, s);
         case 4096: return seg_prep_p<4096>(a, s);
         case 8192: return seg_prep_p<8192>(a, s);
+        case 384: return seg_prep_p<128, 3>(a, s);
+        case 768: return seg_prep_p<256, 3>(a, s);
+        case 1536: return seg_prep_p<512, 3>(a, s);
+        case 3072: return seg_prep_p<1024, 3>(a, s);
         default:
             set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
             return FDAS_ERR_UNSUPPORTED;
@@ -410,6 +502,13 @@ constexpr CorrShape kShapes2048[] = {{128, 8, 0}, {256, 8, 0}, {128, 8, 1}};
 constexpr CorrShape kShapes4096[] = {{512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1}};
 constexpr CorrShape kShapes8192[] = {{512, 8, 0}, {512, 4, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1}};
 constexpr CorrShape kShapes16384[] = {{512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1}};
+// Lengths 3*2^k: three lock-step transforms of L/3 and no split form.  Here one
+// step per workgroup measured fastest (the three transforms' values and their
+// twiddles already fill the registers a cached template row would need).
+constexpr CorrShape kShapes768[] = {{64, 1, 0}, {64, 8, 0}};
+constexpr CorrShape kShapes1536[] = {{64, 1, 0}, {128, 8, 0}};
+constexpr CorrShape kShapes3072[] = {{256, 1, 0}, {128, 1, 0}, {256, 8, 0}};
+constexpr CorrShape kShapes6144[] = {{256, 1, 0}, {512, 1, 0}, {512, 8, 0}, {512, 4, 0}, {256, 8, 0}};
 
 int shape_slot(int L)
 {
@@ -421,6 +520,10 @@ int shape_slot(int L)
         case 4096: return 4;
         case 8192: return 5;
         case 16384: return 6;
+        case 768: return 7;
+        case 1536: return 8;
+        case 3072: return 9;
+        case 6144: return 10;
         default: return -1;
     }
 }
@@ -439,12 +542,16 @@ const CorrShape* shapes_of(int L, int* count)
         SHAPES(4096, kShapes4096)
         SHAPES(8192, kShapes8192)
         SHAPES(16384, kShapes16384)
+        SHAPES(768, kShapes768)
+        SHAPES(1536, kShapes1536)
+        SHAPES(3072, kShapes3072)
+        SHAPES(6144, kShapes6144)
         default: *count = 0; return nullptr;
     }
 #undef SHAPES
 }
 
-std::atomic<int> g_shape[7];
+std::atomic<int> g_shape[11];
 }  // namespace
 
 int corr_variant_count(int L)
@@ -538,6 +645,54 @@ static int corr_l(const CorrArgs& a, hipStream_t s, int v)
     }
 }
 
+template <int M, int T, int G, int P>
+static int corr3_t(const CorrArgs& a, hipStream_t s)
+{
+    using S = typename CStore<P>::T;
+    int err = FDAS_OK;
+    const cplx* tw = fft_twiddles_t(M, T, &err);
+    if (!tw) return err;
+    const cplx* tw3 = fft_r3_twiddles(M, &err);
+    if (!tw3) return err;
+    const double L = 3.0 * M;
+    const float scale = (float)(1.0 / (L * L));
+    const dim3 grid(a.Z, (a.nsteps + G - 1) / G);
+    hipLaunchKernelGGL((corr_r3_kernel<M, G, T, P, float>), grid, dim3(T), 0, s,
+                       static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
+                       a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
+                       static_cast<float*>(a.out), tw, tw3, scale);
+    return check_launch("corr_r3_kernel");
+}
+
+template <int M, int T, int G>
+static int corr3_p(const CorrArgs& a, hipStream_t s)
+{
+    switch (a.prec) {
+        case PREC_F16: return corr3_t<M, T, G, PREC_F16>(a, s);
+        case PREC_BF16: return corr3_t<M, T, G, PREC_BF16>(a, s);
+        case PREC_I16: return corr3_t<M, T, G, PREC_I16>(a, s);
+        default: return corr3_t<M, T, G, PREC_F32>(a, s);
+    }
+}
+
+template <int L, int V = 0>
+static int corr3_l(const CorrArgs& a, hipStream_t s, int v)
+{
+    constexpr const CorrShape* SH = L == 768 ? kShapes768 : L == 1536 ? kShapes1536
+                                  : L == 3072 ? kShapes3072 : kShapes6144;
+    constexpr int NV = L == 768 ? 2 : L == 1536 ? 2 : L == 3072 ? 3 : 5;
+    static_assert(NV == (L == 768 ? sizeof(kShapes768) : L == 1536 ? sizeof(kShapes1536)
+                         : L == 3072 ? sizeof(kShapes3072) : sizeof(kShapes6144)) / sizeof(CorrShape),
+                  "variant count matches the shape table");
+    if constexpr (V < NV) {
+        if (v == V) return corr3_p<L / 3, SH[V].T, SH[V].G>(a, s);
+        return corr3_l<L, V + 1>(a, s, v);
+    } else {
+        set_error("correlation variant %d out of range at length %d", v, L);
+        return FDAS_ERR_INVALID;
+    }
+}
+
 int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
 {
     if (a.nsteps <= 0) return FDAS_OK;
@@ -551,6 +706,10 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
         case 4096: return corr_l<4096>(a, s, v);
         case 8192: return corr_l<8192>(a, s, v);
         case 16384: return corr_l<16384>(a, s, v);
+        case 768: return corr3_l<768>(a, s, v);
+        case 1536: return corr3_l<1536>(a, s, v);
+        case 3072: return corr3_l<3072>(a, s, v);
+        case 6144: return corr3_l<6144>(a, s, v);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;

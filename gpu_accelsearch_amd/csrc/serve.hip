@@ -163,7 +163,7 @@ namespace {
 
 std::vector<int> bank_key(const fdas_params& p)
 {
-    return {p.zmax, p.numharm, p.fftlen, p.uselen, p.precision};
+    return {p.zmax, p.numharm, p.fftlen, p.uselen, p.precision, p.fft_lengths};
 }
 
 // Creates the request's plan on the slot's stream and queues upload + search.

@@ -3,7 +3,9 @@
 // Geometry restates source/bpd_subharmonics.cu:10-67 and headers/bpd_math.h:13.
 // Templates (source/bpd_subharmonics.cu:78-279) are generated on the GPU in FP64,
 // one thread per tap, rounded to FP32, placed wrap-around in rows of length L
-// and transformed in place by the row FFT.
+// and transformed in place by the row FFT (rows of length 3*2^k come out in the
+// decimated order X[3m + r] at r*L/3 + m that the length-3M correlation reads).
+#include <algorithm>
 #include <cmath>
 
 #include "fdas_internal.hpp"
@@ -50,6 +52,32 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax)
     const int bins_needed = (uselen * harmnum) / numharm + 2;
     const int end_effects = 2 * NUMBETWEEN * h_z_response_halfwidth(h_calc_required_z(f, zmax));
     return (int)h_next_pow2(bins_needed + end_effects);
+}
+
+// FFT length of harmonic cur/mx: the reference's power of two, or with
+// FDAS_LENGTHS_SHORTEST its 3/4 (a length 3*2^k in 768..6144) for a sub-harmonic
+// whose every step still fits.  With D = ceil(drhi) - floor(drlo) a step's
+// segment holds numdata = D + 2 bo + 1 bins and its plane row nr = 2 D + 2
+// columns from offset 2 bo, so the normalised bins (numdata <= L/2) and the
+// template span of the last column (2 bo + nr + 2 bo <= L) both fit iff
+// 2 D + 4 bo + 2 <= L.  A step starts at an integer bin and f is dyadic, so D
+// depends on the start only modulo mx: the residues give the worst step.
+int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
+{
+    const int L = h_fft_length(uselen, mx, cur, zmax);
+    if (mode != FDAS_LENGTHS_SHORTEST || cur == mx) return L;
+    const int L3 = 3 * (L / 4);
+    if (L3 < 768 || L3 > 6144) return L;
+    const double f = (double)cur / (double)mx;
+    const int bo = h_z_response_halfwidth((double)h_calc_required_z(f, zmax));
+    const double half = (double)(uselen / 2);
+    int dmax = 0;
+    for (int rho = 0; rho < 2 * mx; ++rho) {
+        const double drlo = h_calc_required_r(f, (double)rho);
+        const double drhi = h_calc_required_r(f, (double)rho + half - DR);
+        dmax = std::max(dmax, (int)(std::ceil(drhi) - std::floor(drlo)));
+    }
+    return 2 * dmax + 4 * bo + 2 <= L3 ? L3 : L;
 }
 
 int h_max_uselen(int fftlen, int zmax)
@@ -147,6 +175,7 @@ int build_template_bank(cplx* d_bank, int zmax_h, int nz, int L, hipStream_t s)
     dim3 grid((unsigned)((4 * hwmax + 255) / 256), (unsigned)nz), block(256);
     hipLaunchKernelGGL(template_rows_kernel, grid, block, 0, s, d_bank, zmax_h, L);
     FDAS_TRY(check_launch("template_rows_kernel"));
+    if (L % 3 == 0) return fft_r3_rows(d_bank, d_bank, L / 3, nz, s);   // decimated order
     if (L <= 16384) return fft_rows(d_bank, d_bank, L, nz, L, L, -1, 1.0f, 0, s);
     cplx* work = nullptr;
     FDAS_HIP(hipMalloc(&work, 2 * (size_t)L * sizeof(cplx)));

@@ -69,6 +69,10 @@ __global__ __launch_bounds__(256) void fill_store_kernel(typename CStore<P>::T* 
 
 unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
 
+// A measured shape replaces the one in use only when it is faster by more than
+// this factor's margin (run-to-run spread of a timing is about 2%).
+constexpr double kTuneMargin = 0.97;
+
 // Fundamental and harmonic FFT lengths of a configuration fit the kernels.
 bool config_valid(int fftlen, int zmax, int numharm)
 {
@@ -269,7 +273,8 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
         const int U = h_max_uselen(L0, base->zmax);
         int cur[MAX_HARM], mx[MAX_HARM];
         h_harmonic_order(base->numharm, cur, mx);
-        for (int h = 0; h < base->numharm; ++h) Ls.push_back(h_fft_length(U, mx[h], cur[h], base->zmax));
+        for (int h = 0; h < base->numharm; ++h)
+            Ls.push_back(h_harmonic_length(U, mx[h], cur[h], base->zmax, base->fft_lengths));
     }
     std::sort(Ls.begin(), Ls.end());
     Ls.erase(std::unique(Ls.begin(), Ls.end()), Ls.end());
@@ -281,10 +286,11 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
         if (corr_variant_count(L) > 1) {
             rc = time_corr_variants(L, base->precision, s, &us);
             if (rc != FDAS_OK) break;
-            int best = 0;
+            // the shape in use stays unless another is faster by the margin
+            int best = corr_variant(L);
             for (int v = 0; v < (int)us.size(); ++v) {
                 if (v < FDAS_TUNE_MAXV) rep->corr_us[k][v] = us[v];
-                if (us[v] < us[best]) best = v;
+                if (us[v] < kTuneMargin * us[best]) best = v;
             }
             set_corr_variant(L, best);
         }
@@ -295,8 +301,9 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
     // 2. harmonic-sum tile shape: whole trials at the first valid candidate length
     for (size_t i = 0; rc == FDAS_OK && i < cand.size() && base->numharm > 1; ++i) {
         if (!config_valid(cand[i], base->zmax, base->numharm)) continue;
-        int bestv = harmsum_variant();
-        double bms = 0.0;
+        const int incumbent = harmsum_variant();
+        int bestv = incumbent;
+        bool timed = false;
         for (int v = 0; v < harmsum_variant_count() && v < FDAS_TUNE_MAXV; ++v) {
             set_harmsum_variant(v);
             double ms = 0.0;
@@ -306,11 +313,11 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
                 break;
             }
             rep->harmsum_ms[v] = ms;
-            if (v == 0 || ms < bms) {
-                bms = ms;
-                bestv = v;
-            }
+            timed = true;
         }
+        // the shape in use stays unless another is faster by the margin
+        for (int v = 0; timed && v < harmsum_variant_count() && v < FDAS_TUNE_MAXV; ++v)
+            if (rep->harmsum_ms[v] > 0.0 && rep->harmsum_ms[v] < kTuneMargin * rep->harmsum_ms[bestv]) bestv = v;
         set_harmsum_variant(bestv);
         rep->harmsum_variant = bestv;
         harmsum_variant_shape(bestv, &rep->harmsum_threads, &rep->harmsum_rows, &rep->harmsum_cols);

@@ -165,7 +165,18 @@ typedef struct fdas_params {
     int max_candidates; /* candidate buffer capacity, 0: 1<<20                */
     float powcut[FDAS_MAX_STAGES]; /* per stage (1,2,4,8,16 harmonics);
                                       <= 0 disables that stage's threshold  */
+    int fft_lengths;    /* FDAS_LENGTHS_SHORTEST (0) or FDAS_LENGTHS_REFERENCE */
 } fdas_params;
+
+/* Sub-harmonic FFT lengths.  The reference rounds every harmonic's length up to
+ * a power of two (source/bpd_subharmonics.cu:10-25).  The correlation outputs
+ * the search keeps do not depend on that length as long as the segment's bins
+ * and the template span of the last kept column fit, so SHORTEST uses 3/4 of
+ * it (a length 3*2^k, 768..6144) for each sub-harmonic where that holds on
+ * every step; REFERENCE keeps the powers of two.  fdas_plan_info.fftlen reports
+ * the lengths used. */
+#define FDAS_LENGTHS_SHORTEST 0
+#define FDAS_LENGTHS_REFERENCE 1
 
 typedef struct fdas_plan_info {
     int uselen;
@@ -344,7 +355,7 @@ int fdas_rz_power(fdas_plan *plan, int trial, const double *r, const double *z, 
  * trials at the first valid candidate length; (3) whole DM trials of base->n samples at
  * each candidate fundamental FFT length (nfftlen 0: 1024..16384), remembering
  * the fastest for plans with fftlen 0 and the same (zmax, numharm, precision). */
-#define FDAS_TUNE_MAX 8
+#define FDAS_TUNE_MAX 12
 #define FDAS_TUNE_MAXV 8
 typedef struct fdas_tune_report {
     int fftlen;                              /* chosen fundamental length        */

@@ -262,3 +262,37 @@ def test_plan_largest_shape(fd, oracle):
     st = synth.steps_covering(400000.3, 2147, 14942)
     print_case("plan 2^25 zmax 600 numharm 16 steps", [0, st])
     _plan_vs_oracle(fd, oracle, x, n, 600, 16, 16384, [0, st])
+
+
+@pytest.mark.parametrize("zmax,numharm,fftlen,want", [
+    (200, 8, 8192, [8192, 4096, 3072, 6144, 1536, 4096, 6144, 8192]),
+    (200, 8, 4096, [4096, 2048, 1536, 3072, 768, 2048, 3072, 4096]),
+    (0, 8, 8192, [8192, 6144, 3072, 8192, 1536, 4096, 6144, 8192]),
+])
+def test_sub_harmonic_lengths(fd, oracle, zmax, numharm, fftlen, want):
+    """Sub-harmonics run at 3/4 of the reference's power-of-two FFT length where
+    every step's bins and template span fit (the kept columns do not depend on
+    the length): both length modes against the oracle and against each other."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 21
+    x = synth.plain_series(n, seed=21, r0=60000.3, z0=-0.6 * zmax, power=90.0)
+    steps = [0, 1, 5, 100]
+    planes = {}
+    for mode in ("shortest", "reference"):
+        print_case("lengths", mode, zmax, numharm, fftlen)
+        P = fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen, fft_lengths=mode)
+        with fd.AccelSearch(P) as s:
+            got = [h[2] for h in s.harmonics]
+            if mode == "shortest":
+                assert got == want, got
+            else:
+                assert all(L & (L - 1) == 0 for L in got), got
+            s.prepare(x)
+            planes[mode] = [s.run_steps(st, 1)[0] for st in steps]
+    ref = oracle.Search(x, n, 6.4e-5, zmax, numharm, fftlen)
+    for i, st in enumerate(steps):
+        want_plane = ref.step_plane(st)
+        for mode in planes:
+            assert power_err(planes[mode][i], want_plane) <= POWER_TOL, (mode, st)
+        assert power_err(planes["shortest"][i], planes["reference"][i].astype(np.float64)) <= 2e-5
+    ref.close()
