@@ -41,7 +41,7 @@ const cplx* fft_twiddles_t(int n, int t, int* err);
 // Radix-3 step table of the length-3m transforms: w^k, w^(2k), w = exp(2 pi i/3m),
 // k in [0, m) (cached per device).
 const cplx* fft_r3_twiddles(int m, int* err);
-// Rows of length 3m (m = 256..2048) forward-transformed into the decimated order:
+// Rows of length 3m (m = 256..4096) forward-transformed into the decimated order:
 // out[row*3m + r*m + j] = DFT_-1(in[row*3m + .])[3j + r].  In place allowed.
 int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s);
 

@@ -204,6 +204,7 @@ int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s)
         case 512: return fft_r3_rows_t<512>(in, out, nrows, s);
         case 1024: return fft_r3_rows_t<1024>(in, out, nrows, s);
         case 2048: return fft_r3_rows_t<2048>(in, out, nrows, s);
+        case 4096: return fft_r3_rows_t<4096>(in, out, nrows, s);
         default:
             set_error("no length-3m row transform with m = %d", m);
             return FDAS_ERR_UNSUPPORTED;

@@ -479,6 +479,7 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
         case 768: return seg_prep_p<256, 3>(a, s);
         case 1536: return seg_prep_p<512, 3>(a, s);
         case 3072: return seg_prep_p<1024, 3>(a, s);
+        case 6144: return seg_prep_p<2048, 3>(a, s);
         default:
             set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
             return FDAS_ERR_UNSUPPORTED;
@@ -509,6 +510,7 @@ constexpr CorrShape kShapes768[] = {{64, 1, 0}, {64, 8, 0}};
 constexpr CorrShape kShapes1536[] = {{64, 1, 0}, {128, 8, 0}};
 constexpr CorrShape kShapes3072[] = {{256, 1, 0}, {128, 1, 0}, {256, 8, 0}};
 constexpr CorrShape kShapes6144[] = {{256, 1, 0}, {512, 1, 0}, {512, 8, 0}, {512, 4, 0}, {256, 8, 0}};
+constexpr CorrShape kShapes12288[] = {{512, 1, 0}, {256, 1, 0}};
 
 int shape_slot(int L)
 {
@@ -524,6 +526,7 @@ int shape_slot(int L)
         case 1536: return 8;
         case 3072: return 9;
         case 6144: return 10;
+        case 12288: return 11;
         default: return -1;
     }
 }
@@ -546,12 +549,13 @@ const CorrShape* shapes_of(int L, int* count)
         SHAPES(1536, kShapes1536)
         SHAPES(3072, kShapes3072)
         SHAPES(6144, kShapes6144)
+        SHAPES(12288, kShapes12288)
         default: *count = 0; return nullptr;
     }
 #undef SHAPES
 }
 
-std::atomic<int> g_shape[11];
+std::atomic<int> g_shape[12];
 }  // namespace
 
 int corr_variant_count(int L)
@@ -679,10 +683,11 @@ template <int L, int V = 0>
 static int corr3_l(const CorrArgs& a, hipStream_t s, int v)
 {
     constexpr const CorrShape* SH = L == 768 ? kShapes768 : L == 1536 ? kShapes1536
-                                  : L == 3072 ? kShapes3072 : kShapes6144;
-    constexpr int NV = L == 768 ? 2 : L == 1536 ? 2 : L == 3072 ? 3 : 5;
+                                  : L == 3072 ? kShapes3072 : L == 6144 ? kShapes6144 : kShapes12288;
+    constexpr int NV = L == 768 ? 2 : L == 1536 ? 2 : L == 3072 ? 3 : L == 6144 ? 5 : 2;
     static_assert(NV == (L == 768 ? sizeof(kShapes768) : L == 1536 ? sizeof(kShapes1536)
-                         : L == 3072 ? sizeof(kShapes3072) : sizeof(kShapes6144)) / sizeof(CorrShape),
+                         : L == 3072 ? sizeof(kShapes3072) : L == 6144 ? sizeof(kShapes6144)
+                         : sizeof(kShapes12288)) / sizeof(CorrShape),
                   "variant count matches the shape table");
     if constexpr (V < NV) {
         if (v == V) return corr3_p<L / 3, SH[V].T, SH[V].G>(a, s);
@@ -710,6 +715,7 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
         case 1536: return corr3_l<1536>(a, s, v);
         case 3072: return corr3_l<3072>(a, s, v);
         case 6144: return corr3_l<6144>(a, s, v);
+        case 12288: return corr3_l<12288>(a, s, v);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;

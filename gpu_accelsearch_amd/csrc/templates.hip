@@ -55,7 +55,7 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax)
 }
 
 // FFT length of harmonic cur/mx: the reference's power of two, or with
-// FDAS_LENGTHS_SHORTEST its 3/4 (a length 3*2^k in 768..6144) for a sub-harmonic
+// FDAS_LENGTHS_SHORTEST its 3/4 (a length 3*2^k in 768..12288) for a sub-harmonic
 // whose every step still fits.  With D = ceil(drhi) - floor(drlo) a step's
 // segment holds numdata = D + 2 bo + 1 bins and its plane row nr = 2 D + 2
 // columns from offset 2 bo, so the normalised bins (numdata <= L/2) and the
@@ -67,7 +67,7 @@ int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
     const int L = h_fft_length(uselen, mx, cur, zmax);
     if (mode != FDAS_LENGTHS_SHORTEST || cur == mx) return L;
     const int L3 = 3 * (L / 4);
-    if (L3 < 768 || L3 > 6144) return L;
+    if (L3 < 768 || L3 > 12288) return L;
     const double f = (double)cur / (double)mx;
     const int bo = h_z_response_halfwidth((double)h_calc_required_z(f, zmax));
     const double half = (double)(uselen / 2);

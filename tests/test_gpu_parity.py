@@ -268,6 +268,8 @@ def test_plan_largest_shape(fd, oracle):
     (200, 8, 8192, [8192, 4096, 3072, 6144, 1536, 4096, 6144, 8192]),
     (200, 8, 4096, [4096, 2048, 1536, 3072, 768, 2048, 3072, 4096]),
     (0, 8, 8192, [8192, 6144, 3072, 8192, 1536, 4096, 6144, 8192]),
+    (600, 16, 16384, [16384, 12288, 4096, 16384, 2048, 6144, 12288, 16384, 1536, 3072, 6144, 8192,
+                      12288, 12288, 16384, 16384]),
 ])
 def test_sub_harmonic_lengths(fd, oracle, zmax, numharm, fftlen, want):
     """Sub-harmonics run at 3/4 of the reference's power-of-two FFT length where

@@ -8,7 +8,7 @@ from conftest import print_case
 pytestmark = pytest.mark.gpu
 
 POWER_TOL = 1e-4
-ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144)
+ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144, 12288)
 
 
 @pytest.fixture(scope="module")
@@ -26,6 +26,7 @@ def fd():
     (1 << 21, 200, 1, 16384),   # 16384
     (1 << 20, 200, 8, 8192),    # 8192, 6144, 4096, 3072, 1536
     (1 << 20, 200, 8, 4096),    # 4096, 3072, 2048, 1536, 768
+    (1 << 21, 600, 2, 16384),   # 16384, 12288
 ])
 def test_every_corr_variant_matches_oracle(fd, oracle, n, zmax, numharm, fftlen):
     from gpu_accelsearch_amd import synth
