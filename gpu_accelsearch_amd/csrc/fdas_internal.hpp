@@ -38,12 +38,22 @@ bool fft_row_supported(long n);
 const cplx* fft_twiddles(int n, int* err);
 // Same for a transform run by t threads (the pass radices depend on n/t).
 const cplx* fft_twiddles_t(int n, int t, int* err);
-// Radix-3 step table of the length-3m transforms: w^k, w^(2k), w = exp(2 pi i/3m),
-// k in [0, m) (cached per device).
-const cplx* fft_r3_twiddles(int m, int* err);
-// Rows of length 3m (m = 256..4096) forward-transformed into the decimated order:
-// out[row*3m + r*m + j] = DFT_-1(in[row*3m + .])[3j + r].  In place allowed.
-int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s);
+// Radix-r step table of the length-r*m transforms (r = 3, 5, 7):
+// w^(q k) at (r-1) k + q - 1, w = exp(2 pi i / (r m)), q in [1, r), k in [0, m)
+// (cached per device).
+const cplx* fft_odd_twiddles(int r, int m, int* err);
+// Rows of length r*m (r = 3: m = 256..4096; r = 5: 256..2048)
+// forward-transformed into the decimated order:
+// out[row*rm + q*m + j] = DFT_-1(in[row*rm + .])[r j + q].  In place allowed.
+int fft_odd_rows(const cplx* in, cplx* out, int r, int m, long nrows, hipStream_t s);
+// The odd factor of an FFT length: r for r * 2^k (r = 3, 5, 7), 1 for a power of two.
+// (The kernels implement r = 3 and 5; the radix-r step itself is generic.)
+inline int odd_factor(int L)
+{
+    for (int r : {3, 5, 7})
+        if (L % r == 0) return r;
+    return 1;
+}
 
 // Row transforms: out[row*out_stride + k] = scale * DFT_dir(in[row*in_stride + .])[k]
 // (times exp(dir 2 pi i (row*k mod tw_m)/tw_m) when tw_m > 0).  In place allowed.
@@ -105,6 +115,8 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax);
 int h_max_uselen(int fftlen, int zmax);
 // FFT length of harmonic cur/mx under FDAS_LENGTHS_* mode (see include/fdas.h).
 int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode);
+// A length R*2^k (R = 3, 5) that the correlation and row kernels implement.
+bool odd_length_supported(int L);
 void h_harmonic_order(int numharm, int* cur, int* max);
 
 // Fill bank[k*L .. (k+1)*L) with the FFT'd template rows z_k = -zmax_h + 2k,

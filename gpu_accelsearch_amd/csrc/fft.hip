@@ -117,7 +117,7 @@ const cplx* fft_twiddles_t(int n, int t, int* err)
     return d;
 }
 
-const cplx* fft_r3_twiddles(int m, int* err)
+const cplx* fft_odd_twiddles(int r, int m, int* err)
 {
     static std::mutex mu;
     static std::map<std::pair<int, int>, cplx*> cache;
@@ -128,21 +128,21 @@ const cplx* fft_r3_twiddles(int m, int* err)
         return nullptr;
     }
     std::lock_guard<std::mutex> lk(mu);
-    auto key = std::make_pair(dev, m);
+    auto key = std::make_pair(dev, r * 65536 + m);
     auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    if (m < 64 || m > 4096 || (m & (m - 1))) {
-        set_error("no length-3m transform with m = %d", m);
+    if ((r != 3 && r != 5 && r != 7) || m < 64 || m > 4096 || (m & (m - 1))) {
+        set_error("no length-%d*%d transform", r, m);
         *err = FDAS_ERR_UNSUPPORTED;
         return nullptr;
     }
-    std::vector<cplx> h(2 * (size_t)m);
-    build_radix3_twiddles(m, h.data());
+    std::vector<cplx> h((size_t)(r - 1) * m);
+    build_odd_twiddles(r, m, h.data());
     cplx* d = nullptr;
     hipError_t e = hipMalloc(&d, h.size() * sizeof(cplx));
     if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        *err = hip_fail(e, "radix-3 twiddle table", __FILE__, __LINE__);
+        *err = hip_fail(e, "radix twiddle table", __FILE__, __LINE__);
         return nullptr;
     }
     cache[key] = d;
@@ -150,63 +150,73 @@ const cplx* fft_r3_twiddles(int m, int* err)
 }
 
 // ------------------------------------------------------------------ kernels
-// Forward transform of rows of length 3M into the decimated order (template
-// banks of the length-3M correlations).  Every thread's loads precede the first
+// Forward transform of rows of length R*M into the decimated order (template
+// banks of the length-R*M correlations).  Every thread's loads precede the first
 // barrier of the transform and its stores follow the last, so in place is safe.
-template <int M>
-__global__ __launch_bounds__(FftCfg<M>::T) void fft_r3_rows_kernel(const cplx* in, cplx* out,
-                                                                    const cplx* __restrict__ tw,
-                                                                    const cplx* __restrict__ tw3)
+template <int R, int M>
+constexpr int odd_rows_threads()
 {
-    constexpr int T = FftCfg<M>::T;
+    return M <= 512 ? 64 : R == 3 && M == 4096 ? 256 : M / 8;
+}
+
+template <int R, int M>
+__global__ __launch_bounds__((odd_rows_threads<R, M>())) void fft_odd_rows_kernel(
+    const cplx* in, cplx* out, const cplx* __restrict__ tw, const cplx* __restrict__ twr)
+{
+    constexpr int T = odd_rows_threads<R, M>();
     using F = BlockFFT<M, T, -1>;
     constexpr int E = F::E;
     static_assert(F::NPASS >= 2, "in-place rows need a barrier between loads and stores");
-    __shared__ cplx lds[3 * F::LDS_ELEMS];
+    __shared__ cplx lds[R * F::LDS_ELEMS];
     const int t = threadIdx.x;
-    const cplx* x = in + (size_t)blockIdx.x * 3 * M;
-    cplx* y = out + (size_t)blockIdx.x * 3 * M;
-    cplx v[3][E];
+    const cplx* x = in + (size_t)blockIdx.x * R * M;
+    cplx* y = out + (size_t)blockIdx.x * R * M;
+    cplx v[R][E];
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < R; ++a)
 #pragma unroll
         for (int i = 0; i < E; ++i) v[a][i] = x[F::in_index(t, i) + M * a];
-    Radix3<M, T>::fwd_split(v, t, tw3);
-    F::template run_multi<3>(v, lds, tw, t);
+    RadixOdd<R, M, T>::fwd_split(v, t, twr);
+    F::template run_multi<R>(v, lds, tw, t);
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+    for (int a = 0; a < R; ++a)
 #pragma unroll
         for (int i = 0; i < E; ++i) y[M * a + F::out_index(t, i)] = v[a][i];
 }
 
-template <int M>
-static int fft_r3_rows_t(const cplx* in, cplx* out, long nrows, hipStream_t s)
+template <int R, int M>
+static int fft_odd_rows_t(const cplx* in, cplx* out, long nrows, hipStream_t s)
 {
+    constexpr int T = odd_rows_threads<R, M>();
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles(M, &err);
+    const cplx* tw = fft_twiddles_t(M, T, &err);
     if (!tw) return err;
-    const cplx* tw3 = fft_r3_twiddles(M, &err);
-    if (!tw3) return err;
-    hipLaunchKernelGGL(fft_r3_rows_kernel<M>, dim3((unsigned)nrows), dim3(FftCfg<M>::T), 0, s, in,
-                       out, tw, tw3);
-    return check_launch("fft_r3_rows_kernel");
+    const cplx* twr = fft_odd_twiddles(R, M, &err);
+    if (!twr) return err;
+    hipLaunchKernelGGL((fft_odd_rows_kernel<R, M>), dim3((unsigned)nrows), dim3(T), 0, s, in, out, tw,
+                       twr);
+    return check_launch("fft_odd_rows_kernel");
 }
 
-int fft_r3_rows(const cplx* in, cplx* out, int m, long nrows, hipStream_t s)
+int fft_odd_rows(const cplx* in, cplx* out, int r, int m, long nrows, hipStream_t s)
 {
     if (nrows <= 0) return FDAS_OK;
     if (nrows > 2147483647L) {
-        set_error("fft_r3_rows: %ld rows exceed the grid limit", nrows);
+        set_error("fft_odd_rows: %ld rows exceed the grid limit", nrows);
         return FDAS_ERR_INVALID;
     }
-    switch (m) {
-        case 256: return fft_r3_rows_t<256>(in, out, nrows, s);
-        case 512: return fft_r3_rows_t<512>(in, out, nrows, s);
-        case 1024: return fft_r3_rows_t<1024>(in, out, nrows, s);
-        case 2048: return fft_r3_rows_t<2048>(in, out, nrows, s);
-        case 4096: return fft_r3_rows_t<4096>(in, out, nrows, s);
+    switch (r * 65536 + m) {
+        case 3 * 65536 + 256: return fft_odd_rows_t<3, 256>(in, out, nrows, s);
+        case 3 * 65536 + 512: return fft_odd_rows_t<3, 512>(in, out, nrows, s);
+        case 3 * 65536 + 1024: return fft_odd_rows_t<3, 1024>(in, out, nrows, s);
+        case 3 * 65536 + 2048: return fft_odd_rows_t<3, 2048>(in, out, nrows, s);
+        case 3 * 65536 + 4096: return fft_odd_rows_t<3, 4096>(in, out, nrows, s);
+        case 5 * 65536 + 256: return fft_odd_rows_t<5, 256>(in, out, nrows, s);
+        case 5 * 65536 + 512: return fft_odd_rows_t<5, 512>(in, out, nrows, s);
+        case 5 * 65536 + 1024: return fft_odd_rows_t<5, 1024>(in, out, nrows, s);
+        case 5 * 65536 + 2048: return fft_odd_rows_t<5, 2048>(in, out, nrows, s);
         default:
-            set_error("no length-3m row transform with m = %d", m);
+            set_error("no length-%d*%d row transform", r, m);
             return FDAS_ERR_UNSUPPORTED;
     }
 }

@@ -298,71 +298,122 @@ struct BlockFFT {
     }
 };
 
-// ------------------------------------------------------------------ length 3M
-// A transform of length N = 3M is three length-M workgroup transforms plus one
-// radix-3 step that each thread applies to its own registers (no extra LDS
-// exchange).  With w = exp(2 pi i / N):
+// ------------------------------------------------------------------ length R*M
+// A transform of length N = R*M (R = 3, 5 or 7) is R length-M workgroup
+// transforms plus one radix-R step that each thread applies to its own
+// registers (no extra LDS exchange).  With w = exp(2 pi i / N):
 //   forward:  x[n1 + M n2] (n1 = t + i T held in x[n2][i]) ->
-//             b_r[n1] = w^(-r n1) DFT3(x[n1], x[n1 + M], x[n1 + 2M])[r],
-//             FFT_M(b_r)[m] = X[3m + r]  ("decimated" order, r-major);
-//   inverse:  V_r[n] = V[3n + r];  Y_r = IFFT_M(V_r);
-//             y[k + M q] = sum_r exp(2 pi i r q / 3) w^(r k) Y_r[k].
-// tw3[2k] = w^k and tw3[2k + 1] = w^(2k) for k in [0, M), rounded from double.
+//             b_r[n1] = w^(-r n1) DFT_R(x[n1 + M n2], n2 < R)[r],
+//             FFT_M(b_r)[m] = X[R m + r]  ("decimated" order, r-major);
+//   inverse:  V_r[n] = V[R n + r];  Y_r = IFFT_M(V_r);
+//             y[k + M q] = sum_r exp(2 pi i r q / R) w^(r k) Y_r[k].
+// tw[(R-1) k + r - 1] = w^(r k), r in [1, R), k in [0, M), rounded from double.
 
-// DFT of three values in place, sign DIR: a, b, c <- y0, y1, y2.
-template <int DIR>
-FDAS_HD void dft3(cplx& a, cplx& b, cplx& c)
+// cos and sin of 2 pi m / R (R = 3, 5, 7), float-rounded constants; after the
+// callers' loops unroll every argument is a compile-time constant.
+FDAS_HD constexpr float odd_cos(int R, int m)
 {
-    constexpr float h = 8.660254038e-01f;   // sqrt(3) / 2
-    const cplx s = b + c, d = b - c;
-    const cplx m = mk(a.x - 0.5f * s.x, a.y - 0.5f * s.y);
-    const cplx u = DIR > 0 ? mk(-h * d.y, h * d.x) : mk(h * d.y, -h * d.x);
-    a = a + s;
-    b = m + u;
-    c = m - u;
+    m %= R;
+    if (m == 0) return 1.0f;
+    if (m > R / 2) m = R - m;
+    return R == 3 ? -5.000000000e-01f
+         : R == 5 ? (m == 1 ? 3.090169944e-01f : -8.090169944e-01f)
+         : (m == 1 ? 6.234898019e-01f : m == 2 ? -2.225209340e-01f : -9.009688679e-01f);
+}
+FDAS_HD constexpr float odd_sin(int R, int m)
+{
+    m %= R;
+    if (m == 0) return 0.0f;
+    const float sg = m > R / 2 ? -1.0f : 1.0f;
+    if (m > R / 2) m = R - m;
+    return sg * (R == 3 ? 8.660254038e-01f
+               : R == 5 ? (m == 1 ? 9.510565163e-01f : 5.877852523e-01f)
+               : (m == 1 ? 7.818314825e-01f : m == 2 ? 9.749279122e-01f : 4.338837391e-01f));
 }
 
-inline void build_radix3_twiddles(int m, cplx* tw3)
+// DFT of R (odd) values in place, sign DIR, from the pair sums and differences
+// s_p = x_p + x_(R-p), d_p = x_p - x_(R-p):
+//   y_q, y_(R-q) = x_0 + sum_p cos(2 pi p q / R) s_p  +/- i DIR sum_p sin(2 pi p q / R) d_p.
+template <int R, int DIR>
+FDAS_HD void dft_odd(cplx* x)
 {
-    for (int k = 0; k < m; ++k) {
-        const double a = 6.283185307179586476925286766559 * (double)k / (3.0 * (double)m);
-        tw3[2 * k] = mk((float)cos(a), (float)sin(a));
-        tw3[2 * k + 1] = mk((float)cos(2.0 * a), (float)sin(2.0 * a));
+    constexpr int H = R / 2;
+    cplx sp[H], dp[H], y[R];
+#pragma unroll
+    for (int p = 1; p <= H; ++p) {
+        sp[p - 1] = x[p] + x[R - p];
+        dp[p - 1] = x[p] - x[R - p];
     }
+    y[0] = x[0];
+#pragma unroll
+    for (int p = 0; p < H; ++p) y[0] = y[0] + sp[p];
+#pragma unroll
+    for (int q = 1; q <= H; ++q) {
+        cplx a = x[0], b = mk(0.f, 0.f);
+#pragma unroll
+        for (int p = 1; p <= H; ++p) {
+            const float c = odd_cos(R, p * q), sn = odd_sin(R, p * q);
+            a = mk(a.x + c * sp[p - 1].x, a.y + c * sp[p - 1].y);
+            b = mk(b.x + sn * dp[p - 1].x, b.y + sn * dp[p - 1].y);
+        }
+        const cplx ib = DIR > 0 ? mk(-b.y, b.x) : mk(b.y, -b.x);
+        y[q] = a + ib;
+        y[R - q] = a - ib;
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[r] = y[r];
 }
 
-template <int M, int T>
-struct Radix3 {
-    static constexpr int N = 3 * M;
+inline void build_odd_twiddles(int R, int m, cplx* tw)
+{
+    for (int k = 0; k < m; ++k)
+        for (int r = 1; r < R; ++r) {
+            const double a = 6.283185307179586476925286766559 * (double)r * (double)k / ((double)R * (double)m);
+            tw[(R - 1) * k + r - 1] = mk((float)cos(a), (float)sin(a));
+        }
+}
+
+template <int R, int M, int T>
+struct RadixOdd {
+    static constexpr int N = R * M;
     static constexpr int E = M / T;
     using FI = BlockFFT<M, T, 1>;
 
-    // Forward radix-3 step before the three forward length-M transforms.
-    FDAS_HD static void fwd_split(cplx (*x)[E], int t, const cplx* tw3)
+    // Forward radix-R step before the R forward length-M transforms.
+    FDAS_HD static void fwd_split(cplx (*x)[E], int t, const cplx* tw)
     {
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int n1 = FI::in_index(t, i);
-            dft3<-1>(x[0][i], x[1][i], x[2][i]);
-            const cplx w1 = tw3[2 * n1], w2 = tw3[2 * n1 + 1];
-            x[1][i] = cmul(x[1][i], mk(w1.x, -w1.y));
-            x[2][i] = cmul(x[2][i], mk(w2.x, -w2.y));
+            cplx u[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) u[r] = x[r][i];
+            dft_odd<R, -1>(u);
+            x[0][i] = u[0];
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const cplx w = tw[(R - 1) * n1 + r - 1];
+                x[r][i] = cmul(u[r], mk(w.x, -w.y));
+            }
         }
     }
-    // Inverse radix-3 step after the three inverse length-M transforms: y[q][i]
+    // Inverse radix-R step after the R inverse length-M transforms: y[q][i]
     // becomes output out_index(t, i) + M q.
-    FDAS_HD static void inv_combine(cplx (*y)[E], int t, const cplx* tw3)
+    FDAS_HD static void inv_combine(cplx (*y)[E], int t, const cplx* tw)
     {
 #pragma unroll
         for (int i = 0; i < E; ++i) {
             const int k = FI::out_index(t, i);
-            y[1][i] = cmul(y[1][i], tw3[2 * k]);
-            y[2][i] = cmul(y[2][i], tw3[2 * k + 1]);
-            dft3<1>(y[0][i], y[1][i], y[2][i]);
+            cplx u[R];
+            u[0] = y[0][i];
+#pragma unroll
+            for (int r = 1; r < R; ++r) u[r] = cmul(y[r][i], tw[(R - 1) * k + r - 1]);
+            dft_odd<R, 1>(u);
+#pragma unroll
+            for (int q = 0; q < R; ++q) y[q][i] = u[q];
         }
     }
-    static void build_twiddles(cplx* tw3) { build_radix3_twiddles(M, tw3); }
+    static void build_twiddles(cplx* tw) { build_odd_twiddles(R, M, tw); }
 };
-
 
 }  // namespace fdas

@@ -74,7 +74,7 @@ using BankKey = std::tuple<int, int, int, int, int, int>;   // device, zmax, num
 static std::mutex g_bank_mu;
 static std::map<BankKey, std::weak_ptr<BankSet>> g_banks;
 
-// Banks of harmonic h have length Ls[h] (a power of two, or 3*2^k stored in the
+// Banks of harmonic h have length Ls[h] (a power of two, or R*2^k stored in the
 // decimated order); the lengths follow from (zmax, H, uselen, mode).
 static int build_banks(int zmax, int H, const int* cur, const int* mx, const int* Ls, int uselen,
                        int prec, int mode, hipStream_t s, std::shared_ptr<BankSet>* out)
@@ -101,7 +101,7 @@ static int build_banks(int zmax, int H, const int* cur, const int* mx, const int
         cplx *bank = nullptr, *split = nullptr;
         FDAS_HIP(hipMalloc(&bank, (size_t)Z * L * sizeof(cplx)));
         FDAS_TRY(build_template_bank(bank, zmax_h, Z, L, s));
-        if (L % 3 == 0) {   // no split form at lengths 3*2^k
+        if (odd_factor(L) > 1) {   // no split form at lengths R*2^k
             const int r1 = store_bank(bank, Z, L, prec, s, &bs->bankq[h], &bs->bank_exp[h], &bs->bytes);
             FDAS_TRY(r1);
             continue;
@@ -401,9 +401,9 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
                 const int lobin = rl - bo;
                 const int hibin = (int)std::ceil(drhi) + bo;
                 const int numdata = hibin - lobin + 1;
-                // power-of-two lengths as the reference; a length 3*2^k must hold
+                // power-of-two lengths as the reference; a length R*2^k must hold
                 // every normalised bin and the template span of the last column
-                const bool fits = pl->L[h] % 3 == 0
+                const bool fits = odd_factor(pl->L[h]) > 1
                                       ? numdata <= pl->L[h] / 2 && nr + 2 * pl->off[h] <= pl->L[h]
                                       : h_next_pow2(numdata) == pl->L[h] / 2;
                 if (!fits) {

@@ -16,6 +16,7 @@
 // Stage path (reference data layout, source/bpd_search.cu:228-360): convolve,
 // powers and add_powers kernels plus the normalise-and-spread kernel used by the
 // reference-named entry points.
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdlib>
@@ -124,23 +125,23 @@ __device__ __forceinline__ float block_max_abs(const cplx* v, float* red, int t)
     return red[T / 64];
 }
 
-// Segment transforms of length R*NH (R = 1, or 3 for the lengths 3*2^k): thread
-// counts per power-of-two part.
+// Segment transforms of length R*NH (R = 1, or 3, 5 for the lengths R*2^k):
+// thread counts per power-of-two part.
 template <int NH, int R>
 constexpr int seg_threads()
 {
-    return R == 1 ? FftCfgS<NH>::T : (NH >= 1024 ? 128 : 64);
+    return R == 1 ? FftCfgS<NH>::T : R >= 5 ? (NH / 4 > 64 ? NH / 4 : 64) : (NH >= 1024 ? 128 : 64);
 }
 
 // Forward transform of one normalised segment of R*NH bins.  Bin n of the
-// segment is held at n = in_index(t, i) + NH*r in v[r][i]; R = 3 adds the
-// radix-3 step, so the output comes in the decimated order A[3m + r] at
-// r*NH + m (what the length-3M correlation reads).
+// segment is held at n = in_index(t, i) + NH*r in v[r][i]; R > 1 adds the
+// radix-R step, so the output comes in the decimated order A[R m + r] at
+// r*NH + m (what the length-R*M correlation reads).
 template <int NH, int R, int P>
 __global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
     const cplx* __restrict__ spec_all, const SegDesc* __restrict__ desc,
     typename CStore<P>::T* __restrict__ seg, long long out_base, int* __restrict__ seg_exp,
-    int eslot_base, const cplx* __restrict__ tw, const cplx* __restrict__ tw3)
+    int eslot_base, const cplx* __restrict__ tw, const cplx* __restrict__ twr)
 {
     constexpr int T = seg_threads<NH, R>();
     using F = BlockFFT<NH, T, -1>;
@@ -189,7 +190,7 @@ __global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
 #pragma unroll
         for (int i = 0; i < E; ++i)
             if (F::in_index(t, i) + NH * r < d.numdata) v[r][i] = scale_nc(v[r][i], c);
-    if constexpr (R == 3) Radix3<NH, T>::fwd_split(v, t, tw3);
+    if constexpr (R > 1) RadixOdd<R, NH, T>::fwd_split(v, t, twr);
     F::template run_multi<R>(v, lds, tw, t);
     typename CStore<P>::T* o = seg + (d.out - out_base);
     if constexpr (P == PREC_F32) {
@@ -237,7 +238,25 @@ __device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
     F::run(v, lds, tw, t);
 }
 
-// Grid (Z, ceil(nsteps/G)): workgroup (z, g) handles steps [g*G, min(g*G+G, nsteps))
+// Workgroup -> (template row z, step group g) of the correlation kernels: rows
+// fastest, over Z rounded up to a multiple of 8.  The hardware hands workgroups to
+// the 8 XCDs round robin (id mod 8), so with the padding XCD k always gets the
+// rows z = k mod 8 and its L2 keeps those template rows across steps (with Z odd
+// the rows would rotate through every XCD and each step re-fetch the bank).  Ids
+// of padding rows return before any barrier.  g0 offsets the step groups of one
+// launch when a chunk needs several.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ bool corr_tile(int Z, int g0, int& z, int& g)
+{
+    const int zp = (Z + kXcds - 1) / kXcds * kXcds;
+    const int id = blockIdx.x;
+    z = id % zp;
+    g = g0 + id / zp;
+    return z < Z;
+}
+
+// Workgroup (z, g) (corr_tile) handles steps [g*G, min(g*G+G, nsteps))
 // with the template row held in registers across them.  Reduced-precision
 // inputs carry per-row / per-segment exponents, folded into the power scale.
 template <int L, int G, int T, int P, typename OutE>
@@ -245,13 +264,15 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
     const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
-    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale)
+    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale, int g0)
 {
     using F = BlockFFT<L, T, 1>;
     using S = typename CStore<P>::T;
     __shared__ cplx lds[F::LDS_ELEMS];
-    const int z = blockIdx.x, t = threadIdx.x;
-    const int s0 = blockIdx.y * G;
+    const int t = threadIdx.x;
+    int z, g;
+    if (!corr_tile(Z, g0, z, g)) return;
+    const int s0 = g * G;
     // G == 1: exactly one step (a loop would let the compiler hoist every twiddle)
     const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
     cplx v[F::E];
@@ -277,39 +298,42 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     }
 }
 
-// Length L = 3M form of the same correlation.  The segment comes in the
-// decimated order A'[r][m] = A[3m + r] (m < M/2) and the template row in the
-// decimated order K'[r][n] = K[3n + r], so with X = [A, A] the product at
-// position 3n + r is A'[r][n mod M/2] * conj K'[r][n]: three length-M inverse
-// transforms in lock step, then the in-register radix-3 step gives y[k + Mq].
-template <int M, int G, int T, int P, typename OutE>
-__global__ __launch_bounds__(T) void corr_r3_kernel(
+// Length L = R*M form of the same correlation (R = 3, 5).  The segment comes
+// in the decimated order A'[r][m] = A[R m + r] (m < M/2) and the template row in
+// the decimated order K'[r][n] = K[R n + r], so with X = [A, A] (period R M/2)
+// the product at position R n + r is A'[r][n mod M/2] * conj K'[r][n]: R
+// length-M inverse transforms in lock step, then the in-register radix-R step
+// gives y[k + M q].
+template <int R, int M, int G, int T, int P, typename OutE>
+__global__ __launch_bounds__(T) void corr_odd_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
     const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
-    OutE* __restrict__ out, const cplx* __restrict__ tw, const cplx* __restrict__ tw3, float scale)
+    OutE* __restrict__ out, const cplx* __restrict__ tw, const cplx* __restrict__ twr, float scale, int g0)
 {
     using F = BlockFFT<M, T, 1>;
     using S = typename CStore<P>::T;
     constexpr int E = F::E, EH = E / 2, MH = M / 2;
-    __shared__ cplx lds[3 * F::LDS_ELEMS];
-    const int z = blockIdx.x, t = threadIdx.x;
-    const int s0 = blockIdx.y * G;
+    __shared__ cplx lds[R * F::LDS_ELEMS];
+    const int t = threadIdx.x;
+    int z, g;
+    if (!corr_tile(Z, g0, z, g)) return;
+    const int s0 = g * G;
     const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
-    const S* K = bank + (size_t)z * 3 * M;
-    [[maybe_unused]] S kr[G > 1 ? 3 : 1][G > 1 ? E : 1];
+    const S* K = bank + (size_t)z * R * M;
+    [[maybe_unused]] S kr[G > 1 ? R : 1][G > 1 ? E : 1];
     if constexpr (G > 1) {
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
+        for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int i = 0; i < E; ++i) kr[r][i] = K[M * r + F::in_index(t, i)];
     }
     const int ek = P == PREC_F32 ? 0 : bank_exp[z];
     for (int s = s0; s < s1; ++s) {
         const S* A = seg + s * seg_stride + seg_off;
-        cplx v[3][E];
+        cplx v[R][E];
 #pragma unroll
-        for (int r = 0; r < 3; ++r) {
+        for (int r = 0; r < R; ++r) {
             cplx a[EH];
 #pragma unroll
             for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[MH * r + t + i * T]);
@@ -319,12 +343,12 @@ __global__ __launch_bounds__(T) void corr_r3_kernel(
                 v[r][i] = cmulc(a[i % EH], CStore<P>::get(kv));
             }
         }
-        F::template run_multi<3>(v, lds, tw, t);
-        Radix3<M, T>::inv_combine(v, t, tw3);
+        F::template run_multi<R>(v, lds, tw, t);
+        RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
         OutE* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
-        for (int q = 0; q < 3; ++q)
+        for (int q = 0; q < R; ++q)
 #pragma unroll
             for (int i = 0; i < E; ++i) {
                 const int col = F::out_index(t, i) + M * q - off;
@@ -346,15 +370,17 @@ __global__ __launch_bounds__(T) void corr_split_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
     const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
-    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale)
+    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale, int g0)
 {
     constexpr int LH = L / 2;
     using F = BlockFFT<LH, T, 1>;
     using S = typename CStore<P>::T;
     constexpr int E = F::E;
     __shared__ cplx lds[F::LDS_ELEMS];
-    const int z = blockIdx.x, t = threadIdx.x;
-    const int s0 = blockIdx.y * G;
+    const int t = threadIdx.x;
+    int z, g;
+    if (!corr_tile(Z, g0, z, g)) return;
+    const int s0 = g * G;
     const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
     const S* Krow = bank + (size_t)z * L;
     [[maybe_unused]] S ke[G > 1 ? E : 1], ko[G > 1 ? E : 1];
@@ -442,14 +468,14 @@ static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
     int err = FDAS_OK;
     const cplx* tw = fft_twiddles_t(NH, T, &err);
     if (!tw) return err;
-    const cplx* tw3 = nullptr;
-    if constexpr (R == 3) {
-        tw3 = fft_r3_twiddles(NH, &err);
-        if (!tw3) return err;
+    const cplx* twr = nullptr;
+    if constexpr (R > 1) {
+        twr = fft_odd_twiddles(R, NH, &err);
+        if (!twr) return err;
     }
     hipLaunchKernelGGL((seg_prep_kernel<NH, R, P>), dim3(a.nseg), dim3(T), 0, s, a.spec, a.desc,
                        static_cast<typename CStore<P>::T*>(a.seg), a.out_base, a.seg_exp,
-                       a.eslot_base, tw, tw3);
+                       a.eslot_base, tw, twr);
     return check_launch("seg_prep_kernel");
 }
 
@@ -480,6 +506,10 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
         case 1536: return seg_prep_p<512, 3>(a, s);
         case 3072: return seg_prep_p<1024, 3>(a, s);
         case 6144: return seg_prep_p<2048, 3>(a, s);
+        case 640: return seg_prep_p<128, 5>(a, s);
+        case 1280: return seg_prep_p<256, 5>(a, s);
+        case 2560: return seg_prep_p<512, 5>(a, s);
+        case 5120: return seg_prep_p<1024, 5>(a, s);
         default:
             set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
             return FDAS_ERR_UNSUPPORTED;
@@ -496,66 +526,66 @@ namespace {
 struct CorrShape {
     int T, G, S;   // threads, steps per workgroup, split form (two L/2 transforms)
 };
-constexpr CorrShape kShapes256[] = {{64, 8, 0}};
-constexpr CorrShape kShapes512[] = {{64, 8, 0}};
-constexpr CorrShape kShapes1024[] = {{64, 8, 0}, {128, 8, 0}};
-constexpr CorrShape kShapes2048[] = {{128, 8, 0}, {256, 8, 0}, {128, 8, 1}};
-constexpr CorrShape kShapes4096[] = {{512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1}};
-constexpr CorrShape kShapes8192[] = {{512, 8, 0}, {512, 4, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1}};
-constexpr CorrShape kShapes16384[] = {{512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1}};
-// Lengths 3*2^k: three lock-step transforms of L/3 and no split form.  Here one
-// step per workgroup measured fastest (the three transforms' values and their
-// twiddles already fill the registers a cached template row would need).
-constexpr CorrShape kShapes768[] = {{64, 1, 0}, {64, 8, 0}};
-constexpr CorrShape kShapes1536[] = {{64, 1, 0}, {128, 8, 0}};
-constexpr CorrShape kShapes3072[] = {{256, 1, 0}, {128, 1, 0}, {256, 8, 0}};
-constexpr CorrShape kShapes6144[] = {{256, 1, 0}, {512, 1, 0}, {512, 8, 0}, {512, 4, 0}, {256, 8, 0}};
-constexpr CorrShape kShapes12288[] = {{512, 1, 0}, {256, 1, 0}};
+template <int L>
+struct Shapes {
+    static constexpr int n = 0;
+};
+#define FDAS_SHAPES(NN, ...)                                               \
+    template <>                                                            \
+    struct Shapes<NN> {                                                    \
+        static constexpr CorrShape v[] = {__VA_ARGS__};                    \
+        static constexpr int n = (int)(sizeof(v) / sizeof(v[0]));          \
+    };
+FDAS_SHAPES(256, {64, 8, 0})
+FDAS_SHAPES(512, {64, 8, 0})
+FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
+FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1})
+FDAS_SHAPES(4096, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
+FDAS_SHAPES(8192, {512, 8, 0}, {512, 4, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
+FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
+// Lengths R*2^k (R = 3, 5): R lock-step transforms of L/R and no split form.
+// Here one step per workgroup measured fastest (the R transforms' values and
+// their twiddles already fill the registers a cached template row would need).
+FDAS_SHAPES(768, {64, 1, 0}, {64, 8, 0})
+FDAS_SHAPES(1536, {64, 1, 0}, {128, 8, 0})
+FDAS_SHAPES(3072, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
+FDAS_SHAPES(6144, {256, 1, 0}, {512, 1, 0}, {512, 8, 0}, {512, 4, 0}, {256, 8, 0})
+FDAS_SHAPES(12288, {512, 1, 0}, {256, 1, 0})
+FDAS_SHAPES(1280, {64, 1, 0})
+FDAS_SHAPES(2560, {128, 1, 0}, {64, 1, 0})
+FDAS_SHAPES(5120, {256, 1, 0}, {128, 1, 0})
+FDAS_SHAPES(10240, {256, 1, 0}, {512, 1, 0})
+#undef FDAS_SHAPES
+
+struct ShapeEntry {
+    int L;
+    const CorrShape* v;
+    int n;
+};
+#define FDAS_ENTRY(NN) {NN, Shapes<NN>::v, Shapes<NN>::n}
+constexpr ShapeEntry kShapeTable[] = {
+    FDAS_ENTRY(256),   FDAS_ENTRY(512),   FDAS_ENTRY(1024),  FDAS_ENTRY(2048),  FDAS_ENTRY(4096),
+    FDAS_ENTRY(8192),  FDAS_ENTRY(16384), FDAS_ENTRY(768),   FDAS_ENTRY(1536),  FDAS_ENTRY(3072),
+    FDAS_ENTRY(6144),  FDAS_ENTRY(12288), FDAS_ENTRY(1280),  FDAS_ENTRY(2560),  FDAS_ENTRY(5120),
+    FDAS_ENTRY(10240)};
+#undef FDAS_ENTRY
+constexpr int kNumLengths = (int)(sizeof(kShapeTable) / sizeof(kShapeTable[0]));
 
 int shape_slot(int L)
 {
-    switch (L) {
-        case 256: return 0;
-        case 512: return 1;
-        case 1024: return 2;
-        case 2048: return 3;
-        case 4096: return 4;
-        case 8192: return 5;
-        case 16384: return 6;
-        case 768: return 7;
-        case 1536: return 8;
-        case 3072: return 9;
-        case 6144: return 10;
-        case 12288: return 11;
-        default: return -1;
-    }
+    for (int k = 0; k < kNumLengths; ++k)
+        if (kShapeTable[k].L == L) return k;
+    return -1;
 }
 
 const CorrShape* shapes_of(int L, int* count)
 {
-#define SHAPES(NN, ARR)                                 \
-    case NN:                                            \
-        *count = (int)(sizeof(ARR) / sizeof(ARR[0]));   \
-        return ARR;
-    switch (L) {
-        SHAPES(256, kShapes256)
-        SHAPES(512, kShapes512)
-        SHAPES(1024, kShapes1024)
-        SHAPES(2048, kShapes2048)
-        SHAPES(4096, kShapes4096)
-        SHAPES(8192, kShapes8192)
-        SHAPES(16384, kShapes16384)
-        SHAPES(768, kShapes768)
-        SHAPES(1536, kShapes1536)
-        SHAPES(3072, kShapes3072)
-        SHAPES(6144, kShapes6144)
-        SHAPES(12288, kShapes12288)
-        default: *count = 0; return nullptr;
-    }
-#undef SHAPES
+    const int k = shape_slot(L);
+    *count = k < 0 ? 0 : kShapeTable[k].n;
+    return k < 0 ? nullptr : kShapeTable[k].v;
 }
 
-std::atomic<int> g_shape[12];
+std::atomic<int> g_shape[kNumLengths];
 }  // namespace
 
 int corr_variant_count(int L)
@@ -592,6 +622,23 @@ void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
     if (split) *split = ok ? sh[v].S : 0;
 }
 
+// Launches of one correlation: step groups [g0, g0 + n) per launch, n chosen so
+// that (Z rounded up to a multiple of 8) * n workgroups of T threads stay below
+// 2^31 work-items.
+template <int T, int G, typename Launch>
+static int corr_launches(const CorrArgs& a, Launch&& launch)
+{
+    const long groups = (a.nsteps + G - 1) / G;
+    const long zr = (a.Z + kXcds - 1) / kXcds;
+    const long per = std::max(1L, (2147483647L / T) / (kXcds * zr));
+    for (long g0 = 0; g0 < groups; g0 += per) {
+        const long n = std::min(per, groups - g0);
+        const int rc = launch(dim3((unsigned)(kXcds * zr * n)), (int)g0);
+        if (rc != FDAS_OK) return rc;
+    }
+    return FDAS_OK;
+}
+
 template <int L, int T, int G, int SPLIT, int P, typename OutE>
 static int corr_t(const CorrArgs& a, hipStream_t s)
 {
@@ -600,23 +647,28 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
     const cplx* tw = fft_twiddles_t(SPLIT ? L / 2 : L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    const dim3 grid(a.Z, (a.nsteps + G - 1) / G);
     if constexpr (SPLIT) {
         if (!a.bank_split) {
             set_error("split correlation at length %d without a split bank", L);
             return FDAS_ERR_INVALID;
         }
-        hipLaunchKernelGGL((corr_split_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
-                           static_cast<const S*>(a.bank_split), a.bank_split_exp, a.Z,
-                           static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H,
-                           a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
-        return check_launch("corr_split_kernel");
+        return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
+            hipLaunchKernelGGL((corr_split_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+                               static_cast<const S*>(a.bank_split), a.bank_split_exp, a.Z,
+                               static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp,
+                               a.H, a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale,
+                               g0);
+            return check_launch("corr_split_kernel");
+        });
     } else {
-        hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
-                           static_cast<const S*>(a.bank), a.bank_exp, a.Z,
-                           static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp, a.H,
-                           a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale);
-        return check_launch("corr_sub_kernel");
+        return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
+            hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+                               static_cast<const S*>(a.bank), a.bank_exp, a.Z,
+                               static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp,
+                               a.H, a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale,
+                               g0);
+            return check_launch("corr_sub_kernel");
+        });
     }
 }
 
@@ -634,14 +686,9 @@ static int corr_p(const CorrArgs& a, hipStream_t s)
 template <int L, int V = 0>
 static int corr_l(const CorrArgs& a, hipStream_t s, int v)
 {
-    constexpr int K = L == 256 ? 0 : L == 512 ? 1 : L == 1024 ? 2 : L == 2048 ? 3 : L == 4096 ? 4
-                    : L == 8192 ? 5 : 6;
-    constexpr const CorrShape* SH = K == 0 ? kShapes256 : K == 1 ? kShapes512 : K == 2 ? kShapes1024
-                                  : K == 3 ? kShapes2048 : K == 4 ? kShapes4096 : K == 5 ? kShapes8192
-                                  : kShapes16384;
-    constexpr int NV = K == 0 ? 1 : K == 1 ? 1 : K == 2 ? 2 : K == 3 ? 3 : K == 4 ? 5 : K == 5 ? 5 : 4;
-    if constexpr (V < NV) {
-        if (v == V) return corr_p<L, SH[V].T, SH[V].G, SH[V].S>(a, s);
+    if constexpr (V < Shapes<L>::n) {
+        constexpr CorrShape sh = Shapes<L>::v[V];
+        if (v == V) return corr_p<L, sh.T, sh.G, sh.S>(a, s);
         return corr_l<L, V + 1>(a, s, v);
     } else {
         set_error("correlation variant %d out of range at length %d", v, L);
@@ -649,49 +696,46 @@ static int corr_l(const CorrArgs& a, hipStream_t s, int v)
     }
 }
 
-template <int M, int T, int G, int P>
-static int corr3_t(const CorrArgs& a, hipStream_t s)
+template <int R, int M, int T, int G, int P>
+static int corr_odd_t(const CorrArgs& a, hipStream_t s)
 {
     using S = typename CStore<P>::T;
     int err = FDAS_OK;
     const cplx* tw = fft_twiddles_t(M, T, &err);
     if (!tw) return err;
-    const cplx* tw3 = fft_r3_twiddles(M, &err);
-    if (!tw3) return err;
-    const double L = 3.0 * M;
+    const cplx* twr = fft_odd_twiddles(R, M, &err);
+    if (!twr) return err;
+    const double L = (double)R * M;
     const float scale = (float)(1.0 / (L * L));
-    const dim3 grid(a.Z, (a.nsteps + G - 1) / G);
-    hipLaunchKernelGGL((corr_r3_kernel<M, G, T, P, float>), grid, dim3(T), 0, s,
-                       static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
-                       a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
-                       static_cast<float*>(a.out), tw, tw3, scale);
-    return check_launch("corr_r3_kernel");
+    return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
+        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float>), grid, dim3(T), 0, s,
+                           static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
+                           a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
+                           static_cast<float*>(a.out), tw, twr, scale, g0);
+        return check_launch("corr_odd_kernel");
+    });
 }
 
-template <int M, int T, int G>
-static int corr3_p(const CorrArgs& a, hipStream_t s)
+template <int R, int M, int T, int G>
+static int corr_odd_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
-        case PREC_F16: return corr3_t<M, T, G, PREC_F16>(a, s);
-        case PREC_BF16: return corr3_t<M, T, G, PREC_BF16>(a, s);
-        case PREC_I16: return corr3_t<M, T, G, PREC_I16>(a, s);
-        default: return corr3_t<M, T, G, PREC_F32>(a, s);
+        case PREC_F16: return corr_odd_t<R, M, T, G, PREC_F16>(a, s);
+        case PREC_BF16: return corr_odd_t<R, M, T, G, PREC_BF16>(a, s);
+        case PREC_I16: return corr_odd_t<R, M, T, G, PREC_I16>(a, s);
+        default: return corr_odd_t<R, M, T, G, PREC_F32>(a, s);
     }
 }
 
 template <int L, int V = 0>
-static int corr3_l(const CorrArgs& a, hipStream_t s, int v)
+static int corr_odd_l(const CorrArgs& a, hipStream_t s, int v)
 {
-    constexpr const CorrShape* SH = L == 768 ? kShapes768 : L == 1536 ? kShapes1536
-                                  : L == 3072 ? kShapes3072 : L == 6144 ? kShapes6144 : kShapes12288;
-    constexpr int NV = L == 768 ? 2 : L == 1536 ? 2 : L == 3072 ? 3 : L == 6144 ? 5 : 2;
-    static_assert(NV == (L == 768 ? sizeof(kShapes768) : L == 1536 ? sizeof(kShapes1536)
-                         : L == 3072 ? sizeof(kShapes3072) : L == 6144 ? sizeof(kShapes6144)
-                         : sizeof(kShapes12288)) / sizeof(CorrShape),
-                  "variant count matches the shape table");
-    if constexpr (V < NV) {
-        if (v == V) return corr3_p<L / 3, SH[V].T, SH[V].G>(a, s);
-        return corr3_l<L, V + 1>(a, s, v);
+    constexpr int R = L % 3 == 0 ? 3 : 5;
+    if constexpr (V < Shapes<L>::n) {
+        constexpr CorrShape sh = Shapes<L>::v[V];
+        static_assert(sh.S == 0, "no split form at lengths R*2^k");
+        if (v == V) return corr_odd_p<R, L / R, sh.T, sh.G>(a, s);
+        return corr_odd_l<L, V + 1>(a, s, v);
     } else {
         set_error("correlation variant %d out of range at length %d", v, L);
         return FDAS_ERR_INVALID;
@@ -711,11 +755,15 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
         case 4096: return corr_l<4096>(a, s, v);
         case 8192: return corr_l<8192>(a, s, v);
         case 16384: return corr_l<16384>(a, s, v);
-        case 768: return corr3_l<768>(a, s, v);
-        case 1536: return corr3_l<1536>(a, s, v);
-        case 3072: return corr3_l<3072>(a, s, v);
-        case 6144: return corr3_l<6144>(a, s, v);
-        case 12288: return corr3_l<12288>(a, s, v);
+        case 768: return corr_odd_l<768>(a, s, v);
+        case 1536: return corr_odd_l<1536>(a, s, v);
+        case 3072: return corr_odd_l<3072>(a, s, v);
+        case 6144: return corr_odd_l<6144>(a, s, v);
+        case 12288: return corr_odd_l<12288>(a, s, v);
+        case 1280: return corr_odd_l<1280>(a, s, v);
+        case 2560: return corr_odd_l<2560>(a, s, v);
+        case 5120: return corr_odd_l<5120>(a, s, v);
+        case 10240: return corr_odd_l<10240>(a, s, v);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;

@@ -3,8 +3,9 @@
 // Geometry restates source/bpd_subharmonics.cu:10-67 and headers/bpd_math.h:13.
 // Templates (source/bpd_subharmonics.cu:78-279) are generated on the GPU in FP64,
 // one thread per tap, rounded to FP32, placed wrap-around in rows of length L
-// and transformed in place by the row FFT (rows of length 3*2^k come out in the
-// decimated order X[3m + r] at r*L/3 + m that the length-3M correlation reads).
+// and transformed in place by the row FFT (rows of length R*2^k, R = 3, 5, 7,
+// come out in the decimated order X[R m + q] at q*L/R + m that the length-R*M
+// correlation reads).
 #include <algorithm>
 #include <cmath>
 
@@ -54,20 +55,27 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax)
     return (int)h_next_pow2(bins_needed + end_effects);
 }
 
-// FFT length of harmonic cur/mx: the reference's power of two, or with
-// FDAS_LENGTHS_SHORTEST its 3/4 (a length 3*2^k in 768..12288) for a sub-harmonic
-// whose every step still fits.  With D = ceil(drhi) - floor(drlo) a step's
-// segment holds numdata = D + 2 bo + 1 bins and its plane row nr = 2 D + 2
-// columns from offset 2 bo, so the normalised bins (numdata <= L/2) and the
-// template span of the last column (2 bo + nr + 2 bo <= L) both fit iff
-// 2 D + 4 bo + 2 <= L.  A step starts at an integer bin and f is dyadic, so D
-// depends on the start only modulo mx: the residues give the worst step.
+// FFT length of harmonic cur/mx: the reference's power of two L, or with
+// FDAS_LENGTHS_SHORTEST the shorter of 5L/8 and 3L/4 (lengths R*2^k, R = 5, 3,
+// that the kernels support) for a sub-harmonic whose every step still fits.
+// (7L/8 as seven lock-step transforms measured slower than L itself.)  With D = ceil(drhi) - floor(drlo) a step's segment holds
+// numdata = D + 2 bo + 1 bins and its plane row nr = 2 D + 2 columns from offset
+// 2 bo, so the normalised bins (numdata <= L/2) and the template span of the last
+// column (2 bo + nr + 2 bo <= L) both fit iff 2 D + 4 bo + 2 <= L.  A step
+// starts at an integer bin and f is dyadic, so D depends on the start only
+// modulo mx: the residues give the worst step.
+bool odd_length_supported(int L)
+{
+    const int r = odd_factor(L);
+    const int m = L / r;
+    if (r == 1 || (m & (m - 1))) return false;
+    return r == 3 ? m >= 256 && m <= 4096 : r == 5 ? m >= 256 && m <= 2048 : false;
+}
+
 int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
 {
     const int L = h_fft_length(uselen, mx, cur, zmax);
-    if (mode != FDAS_LENGTHS_SHORTEST || cur == mx) return L;
-    const int L3 = 3 * (L / 4);
-    if (L3 < 768 || L3 > 12288) return L;
+    if (mode != FDAS_LENGTHS_SHORTEST || cur == mx || L < 1024) return L;
     const double f = (double)cur / (double)mx;
     const int bo = h_z_response_halfwidth((double)h_calc_required_z(f, zmax));
     const double half = (double)(uselen / 2);
@@ -77,7 +85,9 @@ int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
         const double drhi = h_calc_required_r(f, (double)rho + half - DR);
         dmax = std::max(dmax, (int)(std::ceil(drhi) - std::floor(drlo)));
     }
-    return 2 * dmax + 4 * bo + 2 <= L3 ? L3 : L;
+    for (const int c : {5 * (L / 8), 3 * (L / 4)})
+        if (odd_length_supported(c) && 2 * dmax + 4 * bo + 2 <= c) return c;
+    return L;
 }
 
 int h_max_uselen(int fftlen, int zmax)
@@ -175,7 +185,8 @@ int build_template_bank(cplx* d_bank, int zmax_h, int nz, int L, hipStream_t s)
     dim3 grid((unsigned)((4 * hwmax + 255) / 256), (unsigned)nz), block(256);
     hipLaunchKernelGGL(template_rows_kernel, grid, block, 0, s, d_bank, zmax_h, L);
     FDAS_TRY(check_launch("template_rows_kernel"));
-    if (L % 3 == 0) return fft_r3_rows(d_bank, d_bank, L / 3, nz, s);   // decimated order
+    const int r = odd_factor(L);
+    if (r > 1) return fft_odd_rows(d_bank, d_bank, r, L / r, nz, s);   // decimated order
     if (L <= 16384) return fft_rows(d_bank, d_bank, L, nz, L, L, -1, 1.0f, 0, s);
     cplx* work = nullptr;
     FDAS_HIP(hipMalloc(&work, 2 * (size_t)L * sizeof(cplx)));

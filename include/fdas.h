@@ -171,10 +171,10 @@ typedef struct fdas_params {
 /* Sub-harmonic FFT lengths.  The reference rounds every harmonic's length up to
  * a power of two (source/bpd_subharmonics.cu:10-25).  The correlation outputs
  * the search keeps do not depend on that length as long as the segment's bins
- * and the template span of the last kept column fit, so SHORTEST uses 3/4 of
- * it (a length 3*2^k, 768..12288) for each sub-harmonic where that holds on
- * every step; REFERENCE keeps the powers of two.  fdas_plan_info.fftlen reports
- * the lengths used. */
+ * and the template span of the last kept column fit, so SHORTEST uses the
+ * shorter of 5/8 and 3/4 of it (lengths 5*2^k and 3*2^k, 768..12288) for each
+ * sub-harmonic where that holds on every step; REFERENCE keeps the powers of
+ * two.  fdas_plan_info.fftlen reports the lengths used. */
 #define FDAS_LENGTHS_SHORTEST 0
 #define FDAS_LENGTHS_REFERENCE 1
 

@@ -43,78 +43,86 @@ static void sim(const float* in, float* out)
         }
 }
 
-// Length 3M: the three length-M transforms run in lock step (one LDS region
-// each), around the radix-3 step.  Forward reads natural order and writes the
-// decimated order X[3m + r] at r*M + m; inverse reads V[3n + r] and writes y in
-// natural order.  The result is returned in natural order either way.
-template <int M, int T, int DIR, int P>
-static void run_rest3(std::vector<cplx>& v, std::vector<cplx>& lds, const std::vector<cplx>& tw)
+// Length R*M (R odd): the R length-M transforms run in lock step (one LDS
+// region each), around the radix-R step.  Forward reads natural order and
+// writes the decimated order X[R m + r] at r*M + m; inverse reads V[R n + r] and
+// writes y in natural order.  The result is returned in natural order either way.
+template <int R, int M, int T, int DIR, int P>
+static void run_rest_odd(std::vector<cplx>& v, std::vector<cplx>& lds, const std::vector<cplx>& tw)
 {
     using F = BlockFFT<M, T, DIR>;
     constexpr int E = F::E;
     if constexpr (P < F::NPASS) {
         for (int t = 0; t < T; ++t)
-            for (int a = 0; a < 3; ++a)
-                F::template store<P - 1>(&v[((size_t)t * 3 + a) * E], lds.data() + a * F::LDS_ELEMS, t);
+            for (int a = 0; a < R; ++a)
+                F::template store<P - 1>(&v[((size_t)t * R + a) * E], lds.data() + a * F::LDS_ELEMS, t);
         for (int t = 0; t < T; ++t)
-            for (int a = 0; a < 3; ++a)
-                F::template load<P>(&v[((size_t)t * 3 + a) * E], lds.data() + a * F::LDS_ELEMS, t);
+            for (int a = 0; a < R; ++a)
+                F::template load<P>(&v[((size_t)t * R + a) * E], lds.data() + a * F::LDS_ELEMS, t);
         for (int t = 0; t < T; ++t)
-            for (int a = 0; a < 3; ++a) F::template compute<P>(&v[((size_t)t * 3 + a) * E], t, tw.data());
-        run_rest3<M, T, DIR, P + 1>(v, lds, tw);
+            for (int a = 0; a < R; ++a) F::template compute<P>(&v[((size_t)t * R + a) * E], t, tw.data());
+        run_rest_odd<R, M, T, DIR, P + 1>(v, lds, tw);
     }
 }
 
-template <int M, int T, int DIR>
-static void sim3(const float* in, float* out)
+template <int R, int M, int T, int DIR>
+static void sim_odd(const float* in, float* out)
 {
     using F = BlockFFT<M, T, DIR>;
-    using R3 = Radix3<M, T>;
+    using RO = RadixOdd<R, M, T>;
     constexpr int E = F::E;
-    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds(3 * F::LDS_ELEMS), tw3(2 * M);
+    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds(R * F::LDS_ELEMS), twr((R - 1) * M);
     F::build_twiddles(tw.data());
-    R3::build_twiddles(tw3.data());
-    std::vector<cplx> v((size_t)T * 3 * E);
-    auto at = [&](int t, int a, int i) -> cplx& { return v[((size_t)t * 3 + a) * E + i]; };
+    RO::build_twiddles(twr.data());
+    std::vector<cplx> v((size_t)T * R * E);
+    auto at = [&](int t, int a, int i) -> cplx& { return v[((size_t)t * R + a) * E + i]; };
     for (int t = 0; t < T; ++t)
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < R; ++a)
             for (int i = 0; i < E; ++i) {
                 const int n = F::in_index(t, i);
-                const int src = DIR < 0 ? n + M * a : 3 * n + a;
+                const int src = DIR < 0 ? n + M * a : R * n + a;
                 at(t, a, i) = mk(in[2 * src], in[2 * src + 1]);
             }
     if (DIR < 0)
-        for (int t = 0; t < T; ++t) R3::fwd_split(reinterpret_cast<cplx(*)[E]>(&at(t, 0, 0)), t, tw3.data());
+        for (int t = 0; t < T; ++t) RO::fwd_split(reinterpret_cast<cplx(*)[E]>(&at(t, 0, 0)), t, twr.data());
     for (int t = 0; t < T; ++t)
-        for (int a = 0; a < 3; ++a) F::template compute<0>(&at(t, a, 0), t, tw.data());
-    run_rest3<M, T, DIR, 1>(v, lds, tw);
+        for (int a = 0; a < R; ++a) F::template compute<0>(&at(t, a, 0), t, tw.data());
+    run_rest_odd<R, M, T, DIR, 1>(v, lds, tw);
     if (DIR > 0)
-        for (int t = 0; t < T; ++t) R3::inv_combine(reinterpret_cast<cplx(*)[E]>(&at(t, 0, 0)), t, tw3.data());
+        for (int t = 0; t < T; ++t) RO::inv_combine(reinterpret_cast<cplx(*)[E]>(&at(t, 0, 0)), t, twr.data());
     for (int t = 0; t < T; ++t)
-        for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < R; ++a)
             for (int i = 0; i < E; ++i) {
                 const int k = F::out_index(t, i);
-                const int dst = DIR < 0 ? 3 * k + a : k + M * a;
+                const int dst = DIR < 0 ? R * k + a : k + M * a;
                 out[2 * dst] = at(t, a, i).x;
                 out[2 * dst + 1] = at(t, a, i).y;
             }
 }
 
-extern "C" int fft3_sim(int m, int t, int dir, const float* in, float* out)
+extern "C" int fft_odd_sim(int r, int m, int t, int dir, const float* in, float* out)
 {
-#define CASE(MM, TT)                                          \
-    if (m == MM && t == TT) {                                 \
-        if (dir < 0) sim3<MM, TT, -1>(in, out);               \
-        else sim3<MM, TT, +1>(in, out);                       \
+#define CASE(RR, MM, TT)                                      \
+    if (r == RR && m == MM && t == TT) {                      \
+        if (dir < 0) sim_odd<RR, MM, TT, -1>(in, out);        \
+        else sim_odd<RR, MM, TT, +1>(in, out);                \
         return 0;                                             \
     }
-    CASE(128, 64)
-    CASE(256, 64)
-    CASE(512, 64)
-    CASE(1024, 128)
-    CASE(1024, 64)
-    CASE(2048, 256)
-    CASE(2048, 128)
+    CASE(3, 128, 64)
+    CASE(3, 256, 64)
+    CASE(3, 512, 64)
+    CASE(3, 1024, 128)
+    CASE(3, 1024, 64)
+    CASE(3, 2048, 256)
+    CASE(3, 2048, 128)
+    CASE(5, 128, 64)
+    CASE(5, 256, 64)
+    CASE(5, 1024, 128)
+    CASE(5, 1024, 256)
+    CASE(7, 256, 64)
+    CASE(7, 512, 64)
+    CASE(7, 1024, 256)
+    CASE(7, 1024, 128)
 #undef CASE
     return -1;
 }

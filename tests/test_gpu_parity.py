@@ -265,16 +265,17 @@ def test_plan_largest_shape(fd, oracle):
 
 
 @pytest.mark.parametrize("zmax,numharm,fftlen,want", [
-    (200, 8, 8192, [8192, 4096, 3072, 6144, 1536, 4096, 6144, 8192]),
-    (200, 8, 4096, [4096, 2048, 1536, 3072, 768, 2048, 3072, 4096]),
-    (0, 8, 8192, [8192, 6144, 3072, 8192, 1536, 4096, 6144, 8192]),
-    (600, 16, 16384, [16384, 12288, 4096, 16384, 2048, 6144, 12288, 16384, 1536, 3072, 6144, 8192,
-                      12288, 12288, 16384, 16384]),
+    (200, 8, 8192, [8192, 4096, 2560, 6144, 1280, 4096, 5120, 8192]),
+    (200, 8, 4096, [4096, 2048, 1280, 3072, 768, 2048, 2560, 4096]),
+    (0, 8, 8192, [8192, 5120, 2560, 8192, 1280, 4096, 6144, 8192]),
+    (600, 16, 16384, [16384, 10240, 4096, 16384, 2048, 6144, 12288, 16384, 1280, 3072, 5120, 8192,
+                      10240, 12288, 16384, 16384]),
 ])
 def test_sub_harmonic_lengths(fd, oracle, zmax, numharm, fftlen, want):
-    """Sub-harmonics run at 3/4 of the reference's power-of-two FFT length where
-    every step's bins and template span fit (the kept columns do not depend on
-    the length): both length modes against the oracle and against each other."""
+    """Sub-harmonics run at 5/8 or 3/4 of the reference's power-of-two FFT
+    length where every step's bins and template span fit (the kept columns do
+    not depend on the length): both length modes against the oracle and against
+    each other."""
     from gpu_accelsearch_amd import synth
     n = 1 << 21
     x = synth.plain_series(n, seed=21, r0=60000.3, z0=-0.6 * zmax, power=90.0)

@@ -8,7 +8,8 @@ from conftest import print_case
 pytestmark = pytest.mark.gpu
 
 POWER_TOL = 1e-4
-ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144, 12288)
+ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144, 12288, 1280, 2560,
+               5120, 10240)
 
 
 @pytest.fixture(scope="module")
@@ -24,9 +25,10 @@ def fd():
     (1 << 20, 50, 2, 2048),     # correlation lengths 2048 and 1024
     (1 << 20, 200, 2, 8192),    # 8192 and 4096
     (1 << 21, 200, 1, 16384),   # 16384
-    (1 << 20, 200, 8, 8192),    # 8192, 6144, 4096, 3072, 1536
-    (1 << 20, 200, 8, 4096),    # 4096, 3072, 2048, 1536, 768
-    (1 << 21, 600, 2, 16384),   # 16384, 12288
+    (1 << 20, 200, 8, 8192),    # 8192, 6144, 5120, 4096, 2560, 1280
+    (1 << 20, 200, 8, 4096),    # 4096, 3072, 2560, 2048, 1280, 768
+    (1 << 21, 600, 4, 16384),   # 16384, 10240, 4096
+    (1 << 20, 50, 16, 2048),    # 2048, 1536, 1280, 1024, 768, 512, 256
 ])
 def test_every_corr_variant_matches_oracle(fd, oracle, n, zmax, numharm, fftlen):
     from gpu_accelsearch_amd import synth
@@ -67,8 +69,8 @@ def test_autotune_selects_and_applies(fd):
     for l, c in rep["corr"].items():
         us = c["us_per_variant"]
         assert fd.lib().fdas_get_corr_variant(l) == c["variant"]
-        if len(us) > 1:
-            assert us[c["variant"]] == min(us)
+        if len(us) > 1:   # the shape in use stays unless another is 3% faster
+            assert 0.97 * us[c["variant"]] <= min(us)
     with fd.AccelSearch(p) as s:
         assert s.info.fftlen[0] == rep["fftlen"]
 

@@ -176,17 +176,20 @@ def test_workgroup_fft_index_arithmetic(fft_sim, n, t):
         assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
 
 
-@pytest.mark.parametrize("m,t", [(128, 64), (256, 64), (512, 64), (1024, 64), (1024, 128),
-                                 (2048, 128), (2048, 256)])
-def test_length_3m_transform(fft_sim, m, t):
-    """Length 3M = three lock-step length-M workgroup transforms plus the in-register
-    radix-3 step: forward from natural order into the decimated order, inverse back."""
-    n = 3 * m
+@pytest.mark.parametrize("r,m,t", [(3, 128, 64), (3, 256, 64), (3, 512, 64), (3, 1024, 64),
+                                   (3, 1024, 128), (3, 2048, 128), (3, 2048, 256), (5, 128, 64),
+                                   (5, 256, 64), (5, 1024, 128), (5, 1024, 256), (7, 256, 64),
+                                   (7, 512, 64), (7, 1024, 128), (7, 1024, 256)])
+def test_length_rm_transform(fft_sim, r, m, t):
+    """Length R*M (R = 3, 5, 7) = R lock-step length-M workgroup transforms plus the
+    in-register radix-R step: forward from natural order into the decimated
+    order, inverse back."""
+    n = r * m
     rng = np.random.default_rng(n + t)
     for sign in (-1, 1):
         x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
         out = np.zeros(n, np.complex64)
-        rc = fft_sim.fft3_sim(m, t, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
+        rc = fft_sim.fft_odd_sim(r, m, t, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
         assert rc == 0
         ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
         assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
