@@ -15,9 +15,10 @@
 // places x[t + r*T] in v[r]; after the last pass v[b*R + r] holds
 // y[t + b*T + r*(N/R)] (consecutive threads own consecutive outputs).
 //
-// Inter-pass twiddles come from a small table (per-pass blocks [k][r] of
+// Inter-pass twiddles come from a small table (per-pass blocks [r][k] of
 // exp(+2 pi i r k / (Ns R)), built in double on the host) so every twiddle is
-// correctly rounded; DIR < 0 (forward) uses the conjugate.
+// correctly rounded; DIR < 0 (forward) uses the conjugate.  Consecutive threads
+// have consecutive k, so each twiddle load of a wave reads one contiguous run.
 //
 // The phase functions are __host__ __device__ so the index arithmetic can be
 // exercised by a host simulation of the workgroup (tests/test_fft_host_sim.py).
@@ -192,10 +193,10 @@ struct BlockFFT {
             if (Ns > 1) {
                 const int j = t + b * T;
                 const int k = j & (Ns - 1);
-                const cplx* tp = tw + twoff(P) + k * R;
+                const cplx* tp = tw + twoff(P) + k;
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
-                    cplx w = tp[r];
+                    cplx w = tp[r * Ns];
                     if (DIR < 0) w.y = -w.y;
                     v[b * R + r] = cmul(v[b * R + r], w);
                 }
@@ -260,7 +261,7 @@ struct BlockFFT {
             for (int k = 0; k < Ns; ++k)
                 for (int r = 0; r < R; ++r) {
                     const double a = 6.283185307179586476925286766559 * (double)(r * k) / (double)(Ns * R);
-                    tw[twoff(p) + k * R + r] = mk((float)cos(a), (float)sin(a));
+                    tw[twoff(p) + r * Ns + k] = mk((float)cos(a), (float)sin(a));
                 }
         }
     }
