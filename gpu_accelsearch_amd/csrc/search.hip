@@ -519,9 +519,11 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // Launch shapes of the fused correlation per FFT length: (threads, steps per
 // workgroup).  Variant 0 is the default; the autotuner may select another per
 // length.  More threads give more waves per SIMD and fewer values per thread
-// (a smaller radix, so one more LDS pass); fewer steps per workgroup shorten the
-// time template registers stay live.  At 16384 the registers are needed for the
-// transform itself, so every step reloads its template row.
+// (a smaller radix, so one more LDS pass).  More steps per workgroup load the
+// template row and the hoisted twiddles fewer times (16 measured 3-5% faster
+// than 8 at 4096 and 8192) but leave fewer workgroups for short trials.  At
+// 16384 the registers are needed for the transform itself, so every step
+// reloads its template row.
 namespace {
 struct CorrShape {
     int T, G, S;   // threads, steps per workgroup, split form (two L/2 transforms)
@@ -539,9 +541,9 @@ struct Shapes {
 FDAS_SHAPES(256, {64, 8, 0})
 FDAS_SHAPES(512, {64, 8, 0})
 FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
-FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1})
-FDAS_SHAPES(4096, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
-FDAS_SHAPES(8192, {512, 8, 0}, {512, 4, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
+FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0})
+FDAS_SHAPES(4096, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
+FDAS_SHAPES(8192, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5): R lock-step transforms of L/R and no split form.
 // Here one step per workgroup measured fastest (the R transforms' values and
