@@ -465,6 +465,15 @@ def resolve_fftlen(params: SearchParams) -> int:
     return lib().fdas_resolve_fftlen(C.byref(cp))
 
 
+def harmonic_fftlens(params: SearchParams) -> tuple:
+    """(uselen, [FFT length per harmonic]) a plan with ``params`` would use (host only)."""
+    cp = params.to_c()
+    u = C.c_int()
+    arr = (C.c_int * N.MAX_HARM)()
+    check(lib().fdas_harmonic_fftlens(C.byref(cp), C.byref(u), arr), "fdas_harmonic_fftlens")
+    return u.value, list(arr[:params.numharm])
+
+
 # ----------------------------------------------------------------- input files
 PADDING = 2000   # zero floats around readData's samples (headers/bpd_common.h:35)
 

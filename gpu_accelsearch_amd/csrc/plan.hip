@@ -714,6 +714,27 @@ extern "C" int fdas_batch_create(const fdas_params* params, int ntrials, const l
     return plan_create_ex(params, ntrials, lengths, true, nullptr, nullptr, out);
 }
 
+extern "C" int fdas_harmonic_fftlens(const fdas_params* params, int* uselen, int* lengths)
+{
+    if (!params || !uselen || !lengths) {
+        set_error("fdas_harmonic_fftlens: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    fdas_params p = *params;
+    if (p.fftlen == 0) p.fftlen = resolve_fftlen(p);
+    FDAS_TRY(validate(p));
+    const int U = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
+    if (U <= 0 || (U & 1) || U >= 65536 || h_fft_length(U, 1, 1, p.zmax) != p.fftlen) {
+        set_error("uselen %d does not give fundamental FFT length %d at zmax %d", U, p.fftlen, p.zmax);
+        return FDAS_ERR_INVALID;
+    }
+    int cur[MAX_HARM], mx[MAX_HARM];
+    h_harmonic_order(p.numharm, cur, mx);
+    *uselen = U;
+    for (int h = 0; h < p.numharm; ++h) lengths[h] = h_harmonic_length(U, mx[h], cur[h], p.zmax, p.fft_lengths);
+    return FDAS_OK;
+}
+
 extern "C" int fdas_plan_get_info(const fdas_plan* pl, fdas_plan_info* info)
 {
     if (!pl || !info) {

@@ -221,6 +221,10 @@ typedef struct fdas_plan fdas_plan;
 int fdas_plan_create(const fdas_params *params, fdas_plan **plan);
 void fdas_plan_destroy(fdas_plan *plan);
 int fdas_plan_get_info(const fdas_plan *plan, fdas_plan_info *info);
+/* Host-only geometry of a plan with these parameters, without device state: the
+ * overlap-save length U and the FFT length of each of the numharm harmonics
+ * (harmonic order of fdas_plan_info; FDAS_LENGTHS_* mode applied). */
+int fdas_harmonic_fftlens(const fdas_params *params, int *uselen, int lengths[FDAS_MAX_HARM]);
 /* Stream the plan's work is queued on (a hipStream_t; NULL = the plan's own). */
 int fdas_plan_set_stream(fdas_plan *plan, void *hip_stream);
 

@@ -193,3 +193,39 @@ def test_length_rm_transform(fft_sim, r, m, t):
         assert rc == 0
         ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
         assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
+
+
+@pytest.mark.parametrize("zmax,numharm,fftlen,n", [
+    (200, 8, 8192, 1 << 23), (200, 8, 4096, 1 << 23), (0, 8, 8192, 1 << 22), (50, 16, 2048, 1 << 20),
+    (100, 16, 4096, 1 << 21), (600, 16, 16384, 1 << 22), (300, 4, 8192, 1 << 22), (200, 2, 1024, 1 << 20),
+])
+def test_sub_harmonic_lengths_fit_every_step(oracle, zmax, numharm, fftlen, n):
+    """Each sub-harmonic's FFT length under the default (shortest) mode holds every
+    step's normalised bins (numdata <= L/2) and the template span of the last kept
+    column (numrs + 4 binoffset <= L, no wrap-around), by the oracle's own step
+    geometry at every step of a trial.  The reference mode gives the reference's
+    powers of two, and a shorter length is only ever 5/8 or 3/4 of that."""
+    import gpu_accelsearch_amd as fd
+    P = fd.SearchParams(n=n, dt=6.4e-5, zmax=zmax, numharm=numharm, fftlen=fftlen)
+    U, short = fd.harmonic_fftlens(P)
+    U2, ref = fd.harmonic_fftlens(fd.SearchParams(n=n, dt=6.4e-5, zmax=zmax, numharm=numharm,
+                                                  fftlen=fftlen, fft_lengths="reference"))
+    assert U == U2
+    order = oracle.harmonic_order(numharm)
+    for h, (c, m) in enumerate(order):
+        assert ref[h] & (ref[h] - 1) == 0
+        assert short[h] in (ref[h], 5 * ref[h] // 8, 3 * ref[h] // 4), (h, short[h], ref[h])
+    assert short[0] == ref[0] == fftlen
+    rlo = int(np.floor(np.float32(n) * np.float32(6.4e-5)))
+    nsteps = 0
+    startr = float(rlo)
+    while startr + U // 2 < n // 2:
+        nsteps += 1
+        startr += U // 2
+    for k in range(nsteps):
+        g = oracle.step_geometry(numharm, zmax, U, float(rlo + k * (U // 2)))
+        for h in range(1, numharm):
+            assert g[h]["fftlen"] == ref[h]
+            L = short[h]
+            assert g[h]["numdata"] <= L // 2, (k, h, g[h]["numdata"], L)
+            assert g[h]["numrs"] + 4 * g[h]["binoffset"] <= L, (k, h, g[h]["numrs"], g[h]["binoffset"], L)
