@@ -4,9 +4,10 @@
 // shape:
 //  * the launch shape of the fused correlation per FFT length (threads per
 //    workgroup, which fixes the radix of the LDS passes, and steps per workgroup,
-//    the template-block tile): each variant is timed on a block of 64 templates x
-//    64 steps and the fastest is selected process-wide;
-//  * the tile shape of the harmonic sum: whole trials per shape;
+//    the template-block tile): each variant is timed on a block of 128
+//    templates x 256 steps, and one more than 3% faster than the shape in use
+//    replaces it process-wide;
+//  * the tile shape of the harmonic sum: whole trials per shape (same margin);
 //  * the fundamental FFT length (overlap-save segment length): whole DM trials
 //    of the caller's series length are timed for every candidate length, and the
 //    fastest is remembered for plans of the same (zmax, numharm, precision) that
@@ -113,7 +114,8 @@ int fill_store(void* x, long n, int prec, unsigned seed, hipStream_t s)
 // Microseconds per launch of each correlation variant at length L.
 int time_corr_variants(int L, int prec, hipStream_t s, std::vector<double>* us)
 {
-    const int Z = 64, S = 64, W = L - L / 8, off = L / 16;
+    // 128 templates x 256 steps: enough workgroups that 16-step shapes are not starved
+    const int Z = 128, S = 256, W = L - L / 8, off = L / 16;
     const size_t cb = cstore_bytes(prec);
     void *bank = nullptr, *seg = nullptr;
     int *bexp = nullptr, *sexp = nullptr;
