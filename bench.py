@@ -241,8 +241,8 @@ def main() -> int:
                          "(a kernel of known traffic for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
-              "DM trials shard across GPUs with no exchange, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for {args.gpus}); DM trials would "
+              "shard across GPUs, one process each, with no exchange during a search", file=sys.stderr)
         return 2
 
     import gpu_accelsearch_amd as fd
