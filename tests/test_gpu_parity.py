@@ -299,3 +299,30 @@ def test_sub_harmonic_lengths(fd, oracle, zmax, numharm, fftlen, want):
             assert power_err(planes[mode][i], want_plane) <= POWER_TOL, (mode, st)
         assert power_err(planes["shortest"][i], planes["reference"][i].astype(np.float64)) <= 2e-5
     ref.close()
+
+
+def test_length_modes_same_candidates(fd):
+    """A whole production trial (2^23, zmax 200, 8 harmonics, pulsars and red
+    noise) gives the same candidate list under both sub-harmonic length modes,
+    apart from cells within FP32 rounding of a threshold, with powers that agree."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 23
+    x = synth.production_series(n)
+    powcut = (40.0, 45.0, 55.0, 70.0, 0.0)
+    lists = {}
+    for mode in ("shortest", "reference"):
+        print_case("candidates, lengths", mode)
+        P = fd.SearchParams(n=n, zmax=200, numharm=8, fftlen=8192, powcut=powcut, fft_lengths=mode)
+        with fd.AccelSearch(P) as s:
+            s.search(x)
+            c = s.candidates()
+            assert not s.overflowed
+            lists[mode] = {(int(a["step"]), int(a["z_index"]), int(a["r_index"]), int(a["stage"])): float(a["power"])
+                           for a in c}
+    a, b = lists["shortest"], lists["reference"]
+    assert len(a) > 0
+    for key in set(a) ^ set(b):
+        p = a.get(key, b.get(key))
+        assert abs(p - powcut[key[3]]) <= 2e-5 * max(p, 1.0), (key, p)
+    for key in set(a) & set(b):
+        assert abs(a[key] - b[key]) <= 2e-5 * max(b[key], 1.0), (key, a[key], b[key])
