@@ -84,6 +84,7 @@ def test_every_harmsum_shape_is_bit_identical(fd):
     L = fd.lib()
     P = fd.SearchParams(n=n, zmax=100, numharm=16, fftlen=4096, powcut=(30.0, 35.0, 45.0, 60.0, 80.0))
     ref = None
+    default = L.fdas_get_harmsum_variant()
     try:
         with fd.AccelSearch(P) as s:
             s.prepare(x)
@@ -101,4 +102,4 @@ def test_every_harmsum_shape_is_bit_identical(fd):
                     np.testing.assert_array_equal(key[0], ref[0])
                     assert key[1] == ref[1]
     finally:
-        L.fdas_set_harmsum_variant(4)
+        L.fdas_set_harmsum_variant(default)
