@@ -290,6 +290,32 @@ struct BlockFFT {
             passes_multi<NA, P + 1>(v, lds, tw, t);
         }
     }
+    // The same NA transforms through one LDS region: each exchange is done for
+    // one transform after another (more barriers, 1/NA of the LDS).
+    template <int NA, int P>
+    FDAS_HD static void passes_shared(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    {
+        if constexpr (P < NPASS) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                block_sync();
+                store<P - 1>(v[a], lds, t);
+                block_sync();
+                load<P>(v[a], lds, t);
+            }
+#pragma unroll
+            for (int a = 0; a < NA; ++a) compute<P>(v[a], t, tw);
+            passes_shared<NA, P + 1>(v, lds, tw, t);
+        }
+    }
+    template <int NA>
+    FDAS_HD static void run_shared(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    {
+#pragma unroll
+        for (int a = 0; a < NA; ++a) compute<0>(v[a], t, tw);
+        passes_shared<NA, 1>(v, lds, tw, t);
+    }
+
     template <int NA>
     FDAS_HD static void run_multi(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
     {

@@ -304,7 +304,7 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
 // the product at position R n + r is A'[r][n mod M/2] * conj K'[r][n]: R
 // length-M inverse transforms in lock step, then the in-register radix-R step
 // gives y[k + M q].
-template <int R, int M, int G, int T, int P, typename OutE>
+template <int R, int M, int G, int T, int P, typename OutE, bool SHARED>
 __global__ __launch_bounds__(T) void corr_odd_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
@@ -314,7 +314,7 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
     using F = BlockFFT<M, T, 1>;
     using S = typename CStore<P>::T;
     constexpr int E = F::E, EH = E / 2, MH = M / 2;
-    __shared__ cplx lds[R * F::LDS_ELEMS];
+    __shared__ cplx lds[(SHARED ? 1 : R) * F::LDS_ELEMS];
     const int t = threadIdx.x;
     int z, g;
     if (!corr_tile(Z, g0, z, g)) return;
@@ -343,7 +343,8 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
                 v[r][i] = cmulc(a[i % EH], CStore<P>::get(kv));
             }
         }
-        F::template run_multi<R>(v, lds, tw, t);
+        if constexpr (SHARED) F::template run_shared<R>(v, lds, tw, t);
+        else F::template run_multi<R>(v, lds, tw, t);
         RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
         OutE* o = out + ((size_t)s * Z + z) * W;
@@ -526,7 +527,8 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // reloads its template row.
 namespace {
 struct CorrShape {
-    int T, G, S;   // threads, steps per workgroup, split form (two L/2 transforms)
+    int T, G, S;   // threads, steps per workgroup; S: split form (two L/2 transforms)
+                   // at powers of two, one shared LDS region at lengths R*2^k
 };
 template <int L>
 struct Shapes {
@@ -545,18 +547,20 @@ FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0})
 FDAS_SHAPES(4096, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
 FDAS_SHAPES(8192, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
-// Lengths R*2^k (R = 3, 5): R lock-step transforms of L/R and no split form.
-// Here one step per workgroup measured fastest (the R transforms' values and
-// their twiddles already fill the registers a cached template row would need).
-FDAS_SHAPES(768, {64, 1, 0}, {64, 8, 0})
-FDAS_SHAPES(1536, {64, 1, 0}, {128, 8, 0})
-FDAS_SHAPES(3072, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
-FDAS_SHAPES(6144, {256, 1, 0}, {512, 1, 0}, {512, 8, 0}, {512, 4, 0}, {256, 8, 0})
-FDAS_SHAPES(12288, {512, 1, 0}, {256, 1, 0})
-FDAS_SHAPES(1280, {64, 1, 0})
-FDAS_SHAPES(2560, {128, 1, 0}, {64, 1, 0})
-FDAS_SHAPES(5120, {256, 1, 0}, {128, 1, 0})
-FDAS_SHAPES(10240, {256, 1, 0}, {512, 1, 0})
+// Lengths R*2^k (R = 3, 5): R transforms of L/R per step and no split form; S = 1
+// runs their LDS exchanges one transform after another through one region (1/R
+// of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144).
+// One step per workgroup measured fastest here (the R transforms' values and
+// twiddles already take the registers a cached template row would need).
+FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
+FDAS_SHAPES(1536, {64, 1, 1}, {64, 1, 0}, {128, 8, 0})
+FDAS_SHAPES(3072, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
+FDAS_SHAPES(6144, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
+FDAS_SHAPES(12288, {512, 1, 1}, {512, 1, 0}, {256, 1, 0})
+FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
+FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
+FDAS_SHAPES(5120, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
+FDAS_SHAPES(10240, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
 #undef FDAS_SHAPES
 
 struct ShapeEntry {
@@ -621,7 +625,7 @@ void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
     const bool ok = sh && v >= 0 && v < c;
     *threads = ok ? sh[v].T : 0;
     *steps = ok ? sh[v].G : 0;
-    if (split) *split = ok ? sh[v].S : 0;
+    if (split) *split = ok && odd_factor(L) == 1 ? sh[v].S : 0;
 }
 
 // Launches of one correlation: step groups [g0, g0 + n) per launch, n chosen so
@@ -698,7 +702,7 @@ static int corr_l(const CorrArgs& a, hipStream_t s, int v)
     }
 }
 
-template <int R, int M, int T, int G, int P>
+template <int R, int M, int T, int G, int SH, int P>
 static int corr_odd_t(const CorrArgs& a, hipStream_t s)
 {
     using S = typename CStore<P>::T;
@@ -710,7 +714,7 @@ static int corr_odd_t(const CorrArgs& a, hipStream_t s)
     const double L = (double)R * M;
     const float scale = (float)(1.0 / (L * L));
     return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
-        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float>), grid, dim3(T), 0, s,
+        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float, SH != 0>), grid, dim3(T), 0, s,
                            static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
                            a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
                            static_cast<float*>(a.out), tw, twr, scale, g0);
@@ -718,14 +722,14 @@ static int corr_odd_t(const CorrArgs& a, hipStream_t s)
     });
 }
 
-template <int R, int M, int T, int G>
+template <int R, int M, int T, int G, int SH>
 static int corr_odd_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
-        case PREC_F16: return corr_odd_t<R, M, T, G, PREC_F16>(a, s);
-        case PREC_BF16: return corr_odd_t<R, M, T, G, PREC_BF16>(a, s);
-        case PREC_I16: return corr_odd_t<R, M, T, G, PREC_I16>(a, s);
-        default: return corr_odd_t<R, M, T, G, PREC_F32>(a, s);
+        case PREC_F16: return corr_odd_t<R, M, T, G, SH, PREC_F16>(a, s);
+        case PREC_BF16: return corr_odd_t<R, M, T, G, SH, PREC_BF16>(a, s);
+        case PREC_I16: return corr_odd_t<R, M, T, G, SH, PREC_I16>(a, s);
+        default: return corr_odd_t<R, M, T, G, SH, PREC_F32>(a, s);
     }
 }
 
@@ -735,8 +739,7 @@ static int corr_odd_l(const CorrArgs& a, hipStream_t s, int v)
     constexpr int R = L % 3 == 0 ? 3 : 5;
     if constexpr (V < Shapes<L>::n) {
         constexpr CorrShape sh = Shapes<L>::v[V];
-        static_assert(sh.S == 0, "no split form at lengths R*2^k");
-        if (v == V) return corr_odd_p<R, L / R, sh.T, sh.G>(a, s);
+        if (v == V) return corr_odd_p<R, L / R, sh.T, sh.G, sh.S>(a, s);
         return corr_odd_l<L, V + 1>(a, s, v);
     } else {
         set_error("correlation variant %d out of range at length %d", v, L);
