@@ -549,11 +549,12 @@ FDAS_SHAPES(8192, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
-// of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144).
+// of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144;
+// at 1536 a template row kept for 8 steps measured fastest).
 // One step per workgroup measured fastest here (the R transforms' values and
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
-FDAS_SHAPES(1536, {64, 1, 1}, {64, 1, 0}, {128, 8, 0})
+FDAS_SHAPES(1536, {128, 8, 0}, {64, 1, 0}, {64, 1, 1})
 FDAS_SHAPES(3072, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
 FDAS_SHAPES(6144, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
 FDAS_SHAPES(12288, {512, 1, 1}, {512, 1, 0}, {256, 1, 0})
