@@ -6,8 +6,9 @@
 // thresholds after harmonics 1, 3, 7, 15 that the reference leaves as a stub
 // (source/bpd_search.cu:350-356).
 //
-// One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step (4 x 768 by default).  The cells
-// of sub-harmonic h that the tile reads form a rectangle of about (HS_RJ f + 1) x
+// One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step (4 x 896 or
+// 4 x 768 by default).  The cells of sub-harmonic h that the tile reads form a
+// rectangle of about (HS_RJ f + 1) x
 // (HS_CB f + 1) values; the rectangles of as many harmonics as fit are staged in
 // LDS with 16-byte loads and stores (rows start on 4-float boundaries: sub-plane
 // rows are padded to multiples of 4), all loads of a round issued before any
@@ -248,21 +249,25 @@ static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, 
     return check_launch("harmsum_kernel");
 }
 
-// Tile shapes (threads, rows, columns, staged floats).  The default is variant 0
-// (128 threads, 4 rows x 768 columns, 16 KB of LDS): on the production trial it
-// measured 4.63 ms against 4.96 for 2 x 1024, 8.6 against 9.4 ms at 16
-// harmonics, and 4.46 against 5.39 ms at fftlen 4096.  Small LDS tiles keep 4
-// waves per SIMD resident, and more rows per tile reuse more staged sub-harmonic
-// rows.  The autotuner times them all.
+// Tile shapes (threads, rows, columns, staged floats).  The default, variant 0,
+// is 128 threads x 4 rows with 896 or 768 columns, whichever leaves fewer idle
+// columns in the last tile of a U-wide row (U = 7734: 896, 3638: 768).  Small
+// LDS tiles keep 4 waves per SIMD resident, and more rows per tile reuse more
+// staged sub-harmonic rows; on the production trial it measured 4.22 ms against
+// 4.96 for 2 x 1024 tiles (7.8 against 9.4 ms at 16 harmonics, and 4.63 against
+// 5.39 ms at fftlen 4096).  The autotuner times every shape.
 namespace {
 struct HsShape {
-    int T, RJ, CB, LDS;
+    int T, RJ, CB, LDS;   // CB 0: 896 or 768 chosen per launch from U
 };
-constexpr HsShape kHsShapes[] = {{128, 4, 768, 4096}, {128, 3, 1024, 4160}, {128, 2, 1024, 4096},
-                                 {128, 2, 768, 3072}, {128, 4, 512, 4096}, {256, 2, 2048, 8192},
-                                 {256, 4, 1024, 8192}, {128, 3, 768, 3200}};
+constexpr HsShape kHsShapes[] = {{128, 4, 0, 4608}, {128, 4, 768, 4096}, {128, 4, 896, 4608},
+                                 {128, 3, 1024, 4160}, {128, 2, 1024, 4096}, {128, 3, 896, 3840},
+                                 {256, 2, 2048, 8192}, {128, 5, 640, 4096}};
 constexpr int kHsCount = (int)(sizeof(kHsShapes) / sizeof(kHsShapes[0]));
 std::atomic<int> g_hs_variant{0};
+
+// Idle columns in the last tile of a row of U columns with tiles of cb.
+int tail_waste(int U, int cb) { return (U + cb - 1) / cb * cb - U; }
 }  // namespace
 
 int harmsum_variant_count() { return kHsCount; }
@@ -292,14 +297,16 @@ int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, in
     const int st = store_plane ? 1 : 0;
 #define HS_ARGS d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, st, s
     switch (g_hs_variant.load()) {
-        case 1: return harmsum_t<128, 3, 1024, 4160>(HS_ARGS);
-        case 2: return harmsum_t<128, 2, 1024, 4096>(HS_ARGS);
-        case 3: return harmsum_t<128, 2, 768, 3072>(HS_ARGS);
-        case 4: return harmsum_t<128, 4, 512, 4096>(HS_ARGS);
-        case 5: return harmsum_t<256, 2, 2048, 8192>(HS_ARGS);
-        case 6: return harmsum_t<256, 4, 1024, 8192>(HS_ARGS);
-        case 7: return harmsum_t<128, 3, 768, 3200>(HS_ARGS);
-        default: return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
+        case 1: return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
+        case 2: return harmsum_t<128, 4, 896, 4608>(HS_ARGS);
+        case 3: return harmsum_t<128, 3, 1024, 4160>(HS_ARGS);
+        case 4: return harmsum_t<128, 2, 1024, 4096>(HS_ARGS);
+        case 5: return harmsum_t<128, 3, 896, 3840>(HS_ARGS);
+        case 6: return harmsum_t<256, 2, 2048, 8192>(HS_ARGS);
+        case 7: return harmsum_t<128, 5, 640, 4096>(HS_ARGS);
+        default:
+            if (tail_waste(hs.U, 896) <= tail_waste(hs.U, 768)) return harmsum_t<128, 4, 896, 4608>(HS_ARGS);
+            return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
     }
 #undef HS_ARGS
 }

@@ -374,7 +374,7 @@ typedef struct fdas_tune_report {
     int corr_split[FDAS_TUNE_MAX];           /* 1: two L/2 transforms per step    */
     double corr_us[FDAS_TUNE_MAX][FDAS_TUNE_MAXV]; /* us per launch per variant (0: n/a) */
     int harmsum_variant;                     /* selected harmonic-sum tile shape   */
-    int harmsum_threads, harmsum_rows, harmsum_cols;
+    int harmsum_threads, harmsum_rows, harmsum_cols;  /* cols 0: 896 or 768, by row width */
     double harmsum_ms[FDAS_TUNE_MAXV];       /* ms per trial per shape (0: n/a)    */
 } fdas_tune_report;
 
