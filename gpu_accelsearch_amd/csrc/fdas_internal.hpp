@@ -42,12 +42,12 @@ const cplx* fft_twiddles_t(int n, int t, int* err);
 // w^(q k) at (r-1) k + q - 1, w = exp(2 pi i / (r m)), q in [1, r), k in [0, m)
 // (cached per device).
 const cplx* fft_odd_twiddles(int r, int m, int* err);
-// Rows of length r*m (r = 3: m = 256..4096; r = 5: 256..2048)
+// Rows of length r*m (r = 3: m = 256..4096; r = 5: 256..2048; r = 7: 1024)
 // forward-transformed into the decimated order:
 // out[row*rm + q*m + j] = DFT_-1(in[row*rm + .])[r j + q].  In place allowed.
 int fft_odd_rows(const cplx* in, cplx* out, int r, int m, long nrows, hipStream_t s);
 // The odd factor of an FFT length: r for r * 2^k (r = 3, 5, 7), 1 for a power of two.
-// (The kernels implement r = 3 and 5; the radix-r step itself is generic.)
+// (The kernels implement r = 3, 5 and 7 x 1024; the radix-r step itself is generic.)
 inline int odd_factor(int L)
 {
     for (int r : {3, 5, 7})
@@ -115,7 +115,7 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax);
 int h_max_uselen(int fftlen, int zmax);
 // FFT length of harmonic cur/mx under FDAS_LENGTHS_* mode (see include/fdas.h).
 int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode);
-// A length R*2^k (R = 3, 5) that the correlation and row kernels implement.
+// A length R*2^k (R = 3, 5; 7 at 7168) that the correlation and row kernels implement.
 bool odd_length_supported(int L);
 void h_harmonic_order(int numharm, int* cur, int* max);
 

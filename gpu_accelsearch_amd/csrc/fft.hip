@@ -215,6 +215,7 @@ int fft_odd_rows(const cplx* in, cplx* out, int r, int m, long nrows, hipStream_
         case 5 * 65536 + 512: return fft_odd_rows_t<5, 512>(in, out, nrows, s);
         case 5 * 65536 + 1024: return fft_odd_rows_t<5, 1024>(in, out, nrows, s);
         case 5 * 65536 + 2048: return fft_odd_rows_t<5, 2048>(in, out, nrows, s);
+        case 7 * 65536 + 1024: return fft_odd_rows_t<7, 1024>(in, out, nrows, s);
         default:
             set_error("no length-%d*%d row transform", r, m);
             return FDAS_ERR_UNSUPPORTED;

@@ -125,7 +125,7 @@ __device__ __forceinline__ float block_max_abs(const cplx* v, float* red, int t)
     return red[T / 64];
 }
 
-// Segment transforms of length R*NH (R = 1, or 3, 5 for the lengths R*2^k):
+// Segment transforms of length R*NH (R = 1, or 3, 5, 7 for the lengths R*2^k):
 // thread counts per power-of-two part.
 template <int NH, int R>
 constexpr int seg_threads()
@@ -298,7 +298,7 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     }
 }
 
-// Length L = R*M form of the same correlation (R = 3, 5).  The segment comes
+// Length L = R*M form of the same correlation (R = 3, 5, 7).  The segment comes
 // in the decimated order A'[r][m] = A[R m + r] (m < M/2) and the template row in
 // the decimated order K'[r][n] = K[R n + r], so with X = [A, A] (period R M/2)
 // the product at position R n + r is A'[r][n mod M/2] * conj K'[r][n]: R
@@ -511,6 +511,7 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
         case 1280: return seg_prep_p<256, 5>(a, s);
         case 2560: return seg_prep_p<512, 5>(a, s);
         case 5120: return seg_prep_p<1024, 5>(a, s);
+        case 3584: return seg_prep_p<512, 7>(a, s);
         default:
             set_error("segment transform length %d unsupported (FFT length %d)", L / 2, L);
             return FDAS_ERR_UNSUPPORTED;
@@ -547,7 +548,7 @@ FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0})
 FDAS_SHAPES(4096, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
 FDAS_SHAPES(8192, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
-// Lengths R*2^k (R = 3, 5): R transforms of L/R per step and no split form; S = 1
+// Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
 // of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144;
 // at 1536 a template row kept for 8 steps measured fastest).
@@ -562,6 +563,7 @@ FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
 FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
 FDAS_SHAPES(5120, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
 FDAS_SHAPES(10240, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
+FDAS_SHAPES(7168, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
 #undef FDAS_SHAPES
 
 struct ShapeEntry {
@@ -574,7 +576,7 @@ constexpr ShapeEntry kShapeTable[] = {
     FDAS_ENTRY(256),   FDAS_ENTRY(512),   FDAS_ENTRY(1024),  FDAS_ENTRY(2048),  FDAS_ENTRY(4096),
     FDAS_ENTRY(8192),  FDAS_ENTRY(16384), FDAS_ENTRY(768),   FDAS_ENTRY(1536),  FDAS_ENTRY(3072),
     FDAS_ENTRY(6144),  FDAS_ENTRY(12288), FDAS_ENTRY(1280),  FDAS_ENTRY(2560),  FDAS_ENTRY(5120),
-    FDAS_ENTRY(10240)};
+    FDAS_ENTRY(10240), FDAS_ENTRY(7168)};
 #undef FDAS_ENTRY
 constexpr int kNumLengths = (int)(sizeof(kShapeTable) / sizeof(kShapeTable[0]));
 
@@ -737,7 +739,7 @@ static int corr_odd_p(const CorrArgs& a, hipStream_t s)
 template <int L, int V = 0>
 static int corr_odd_l(const CorrArgs& a, hipStream_t s, int v)
 {
-    constexpr int R = L % 3 == 0 ? 3 : 5;
+    constexpr int R = L % 3 == 0 ? 3 : L % 5 == 0 ? 5 : 7;
     if constexpr (V < Shapes<L>::n) {
         constexpr CorrShape sh = Shapes<L>::v[V];
         if (v == V) return corr_odd_p<R, L / R, sh.T, sh.G, sh.S>(a, s);
@@ -770,6 +772,7 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant)
         case 2560: return corr_odd_l<2560>(a, s, v);
         case 5120: return corr_odd_l<5120>(a, s, v);
         case 10240: return corr_odd_l<10240>(a, s, v);
+        case 7168: return corr_odd_l<7168>(a, s, v);
         default:
             set_error("fused correlation length %d unsupported", L);
             return FDAS_ERR_UNSUPPORTED;

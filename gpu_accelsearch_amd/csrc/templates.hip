@@ -56,9 +56,9 @@ int h_fft_length(int uselen, int numharm, int harmnum, int zmax)
 }
 
 // FFT length of harmonic cur/mx: the reference's power of two L, or with
-// FDAS_LENGTHS_SHORTEST the shorter of 5L/8 and 3L/4 (lengths R*2^k, R = 5, 3,
-// that the kernels support) for a sub-harmonic whose every step still fits.
-// (7L/8 as seven lock-step transforms measured slower than L itself.)  With D = ceil(drhi) - floor(drlo) a step's segment holds
+// FDAS_LENGTHS_SHORTEST the shortest of 5L/8, 3L/4 and 7L/8 (lengths R*2^k, R = 5,
+// 3, 7, that the kernels support: 7L/8 only as 7168) for a sub-harmonic whose
+// every step still fits.  With D = ceil(drhi) - floor(drlo) a step's segment holds
 // numdata = D + 2 bo + 1 bins and its plane row nr = 2 D + 2 columns from offset
 // 2 bo, so the normalised bins (numdata <= L/2) and the template span of the last
 // column (2 bo + nr + 2 bo <= L) both fit iff 2 D + 4 bo + 2 <= L.  A step
@@ -69,7 +69,8 @@ bool odd_length_supported(int L)
     const int r = odd_factor(L);
     const int m = L / r;
     if (r == 1 || (m & (m - 1))) return false;
-    return r == 3 ? m >= 256 && m <= 4096 : r == 5 ? m >= 256 && m <= 2048 : false;
+    // radix 7 only at 7 x 1024: 3584 measured slower than 4096
+    return r == 3 ? m >= 256 && m <= 4096 : r == 5 ? m >= 256 && m <= 2048 : m == 1024;
 }
 
 int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
@@ -85,7 +86,7 @@ int h_harmonic_length(int uselen, int mx, int cur, int zmax, int mode)
         const double drhi = h_calc_required_r(f, (double)rho + half - DR);
         dmax = std::max(dmax, (int)(std::ceil(drhi) - std::floor(drlo)));
     }
-    for (const int c : {5 * (L / 8), 3 * (L / 4)})
+    for (const int c : {5 * (L / 8), 3 * (L / 4), 7 * (L / 8)})
         if (odd_length_supported(c) && 2 * dmax + 4 * bo + 2 <= c) return c;
     return L;
 }

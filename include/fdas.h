@@ -172,9 +172,9 @@ typedef struct fdas_params {
  * a power of two (source/bpd_subharmonics.cu:10-25).  The correlation outputs
  * the search keeps do not depend on that length as long as the segment's bins
  * and the template span of the last kept column fit, so SHORTEST uses the
- * shorter of 5/8 and 3/4 of it (lengths 5*2^k and 3*2^k, 768..12288) for each
- * sub-harmonic where that holds on every step; REFERENCE keeps the powers of
- * two.  fdas_plan_info.fftlen reports the lengths used. */
+ * shortest of 5/8, 3/4 and 7/8 of it (lengths 5*2^k and 3*2^k in 768..12288, and
+ * 7168) for each sub-harmonic where that holds on every step; REFERENCE keeps
+ * the powers of two.  fdas_plan_info.fftlen reports the lengths used. */
 #define FDAS_LENGTHS_SHORTEST 0
 #define FDAS_LENGTHS_REFERENCE 1
 

@@ -265,14 +265,14 @@ def test_plan_largest_shape(fd, oracle):
 
 
 @pytest.mark.parametrize("zmax,numharm,fftlen,want", [
-    (200, 8, 8192, [8192, 4096, 2560, 6144, 1280, 4096, 5120, 8192]),
+    (200, 8, 8192, [8192, 4096, 2560, 6144, 1280, 4096, 5120, 7168]),
     (200, 8, 4096, [4096, 2048, 1280, 3072, 768, 2048, 2560, 4096]),
-    (0, 8, 8192, [8192, 5120, 2560, 8192, 1280, 4096, 6144, 8192]),
-    (600, 16, 16384, [16384, 10240, 4096, 16384, 2048, 6144, 12288, 16384, 1280, 3072, 5120, 8192,
+    (0, 8, 8192, [8192, 5120, 2560, 7168, 1280, 4096, 6144, 8192]),
+    (600, 16, 16384, [16384, 10240, 4096, 16384, 2048, 6144, 12288, 16384, 1280, 3072, 5120, 7168,
                       10240, 12288, 16384, 16384]),
 ])
 def test_sub_harmonic_lengths(fd, oracle, zmax, numharm, fftlen, want):
-    """Sub-harmonics run at 5/8 or 3/4 of the reference's power-of-two FFT
+    """Sub-harmonics run at 5/8, 3/4 or 7/8 of the reference's power-of-two FFT
     length where every step's bins and template span fit (the kept columns do
     not depend on the length): both length modes against the oracle and against
     each other."""

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 POWER_TOL = 1e-4
 ALL_LENGTHS = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144, 12288, 1280, 2560,
-               5120, 10240)
+               5120, 10240, 7168)
 
 
 @pytest.fixture(scope="module")
@@ -25,7 +25,7 @@ def fd():
     (1 << 20, 50, 2, 2048),     # correlation lengths 2048 and 1024
     (1 << 20, 200, 2, 8192),    # 8192 and 4096
     (1 << 21, 200, 1, 16384),   # 16384
-    (1 << 20, 200, 8, 8192),    # 8192, 6144, 5120, 4096, 2560, 1280
+    (1 << 20, 200, 8, 8192),    # 8192, 7168, 6144, 5120, 4096, 2560, 1280
     (1 << 20, 200, 8, 4096),    # 4096, 3072, 2560, 2048, 1280, 768
     (1 << 21, 600, 4, 16384),   # 16384, 10240, 4096
     (1 << 20, 50, 16, 2048),    # 2048, 1536, 1280, 1024, 768, 512, 256

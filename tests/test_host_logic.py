@@ -204,7 +204,7 @@ def test_sub_harmonic_lengths_fit_every_step(oracle, zmax, numharm, fftlen, n):
     step's normalised bins (numdata <= L/2) and the template span of the last kept
     column (numrs + 4 binoffset <= L, no wrap-around), by the oracle's own step
     geometry at every step of a trial.  The reference mode gives the reference's
-    powers of two, and a shorter length is only ever 5/8 or 3/4 of that."""
+    powers of two, and a shorter length is only ever 5/8, 3/4 or 7/8 of that."""
     import gpu_accelsearch_amd as fd
     P = fd.SearchParams(n=n, dt=6.4e-5, zmax=zmax, numharm=numharm, fftlen=fftlen)
     U, short = fd.harmonic_fftlens(P)
@@ -214,7 +214,7 @@ def test_sub_harmonic_lengths_fit_every_step(oracle, zmax, numharm, fftlen, n):
     order = oracle.harmonic_order(numharm)
     for h, (c, m) in enumerate(order):
         assert ref[h] & (ref[h] - 1) == 0
-        assert short[h] in (ref[h], 5 * ref[h] // 8, 3 * ref[h] // 4), (h, short[h], ref[h])
+        assert short[h] in (ref[h], 5 * ref[h] // 8, 3 * ref[h] // 4, 7 * ref[h] // 8), (h, short[h], ref[h])
     assert short[0] == ref[0] == fftlen
     rlo = int(np.floor(np.float32(n) * np.float32(6.4e-5)))
     nsteps = 0
