@@ -223,7 +223,7 @@ __device__ __forceinline__ void load_template(const typename CStore<P>::T* __res
 
 // y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.  Position t + i*T needs
 // A[t + (i mod E/2)*T], so each thread loads E/2 segment values.
-template <int L, int T = FftCfgS<L>::T, int P = PREC_F32>
+template <int L, int T = FftCfgS<L>::T, int P = PREC_F32, bool DB = false>
 __device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
                                           const typename CStore<P>::T* __restrict__ A,
                                           const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
@@ -235,7 +235,8 @@ __device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
     for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[t + i * T]);
 #pragma unroll
     for (int i = 0; i < F::E; ++i) v[i] = cmulc(a[i % EH], CStore<P>::get(kr[i]));
-    F::run(v, lds, tw, t);
+    if constexpr (DB) F::run2(v, lds, tw, t);
+    else F::run(v, lds, tw, t);
 }
 
 // Workgroup -> (template row z, step group g) of the correlation kernels: rows
@@ -259,7 +260,9 @@ __device__ __forceinline__ bool corr_tile(int Z, int g0, int& z, int& g)
 // Workgroup (z, g) (corr_tile) handles steps [g*G, min(g*G+G, nsteps))
 // with the template row held in registers across them.  Reduced-precision
 // inputs carry per-row / per-segment exponents, folded into the power scale.
-template <int L, int G, int T, int P, typename OutE>
+// DB: the transform's exchanges alternate between two LDS regions (one barrier
+// each instead of two).
+template <int L, int G, int T, int P, typename OutE, bool DB>
 __global__ __launch_bounds__(T) void corr_sub_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
 {
     using F = BlockFFT<L, T, 1>;
     using S = typename CStore<P>::T;
-    __shared__ cplx lds[F::LDS_ELEMS];
+    __shared__ cplx lds[(DB ? 2 : 1) * F::LDS_ELEMS];
     const int t = threadIdx.x;
     int z, g;
     if (!corr_tile(Z, g0, z, g)) return;
@@ -281,11 +284,11 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     const int ek = P == PREC_F32 ? 0 : bank_exp[z];
     for (int s = s0; s < s1; ++s) {
         if constexpr (G > 1) {
-            corr_ifft<L, T, P>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T, P, DB>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
         } else {
             S k1[F::E];
             load_template<L, T, P>(bank + (size_t)z * L, k1, t);
-            corr_ifft<L, T, P>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T, P, DB>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
         }
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
         OutE* o = out + ((size_t)s * Z + z) * W;
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
 // the product at position R n + r is A'[r][n mod M/2] * conj K'[r][n]: R
 // length-M inverse transforms in lock step, then the in-register radix-R step
 // gives y[k + M q].
-template <int R, int M, int G, int T, int P, typename OutE, bool SHARED>
+template <int R, int M, int G, int T, int P, typename OutE, int SH>
 __global__ __launch_bounds__(T) void corr_odd_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
@@ -314,7 +317,7 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
     using F = BlockFFT<M, T, 1>;
     using S = typename CStore<P>::T;
     constexpr int E = F::E, EH = E / 2, MH = M / 2;
-    __shared__ cplx lds[(SHARED ? 1 : R) * F::LDS_ELEMS];
+    __shared__ cplx lds[(SH == 0 ? R : SH) * F::LDS_ELEMS];
     const int t = threadIdx.x;
     int z, g;
     if (!corr_tile(Z, g0, z, g)) return;
@@ -343,7 +346,8 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
                 v[r][i] = cmulc(a[i % EH], CStore<P>::get(kv));
             }
         }
-        if constexpr (SHARED) F::template run_shared<R>(v, lds, tw, t);
+        if constexpr (SH == 1) F::template run_shared<R>(v, lds, tw, t);
+        else if constexpr (SH == 2) F::template run_shared2<R>(v, lds, tw, t);
         else F::template run_multi<R>(v, lds, tw, t);
         RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
@@ -525,11 +529,16 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // template row and the hoisted twiddles fewer times (16 measured 3-5% faster
 // than 8 at 4096 and 8192) but leave fewer workgroups for short trials.  At
 // 16384 the registers are needed for the transform itself, so every step
-// reloads its template row.
+// reloads its template row.  Two alternating LDS regions (S = 2) halve the
+// barriers of each exchange: 7% faster at 4096 and 2% at 8192 (512 threads,
+// one workgroup per CU, where every barrier idles the CU), slower with 1024
+// threads.
 namespace {
 struct CorrShape {
-    int T, G, S;   // threads, steps per workgroup; S: split form (two L/2 transforms)
-                   // at powers of two, one shared LDS region at lengths R*2^k
+    int T, G, S;   // threads, steps per workgroup; S at powers of two: 1 split form
+                   // (two L/2 transforms), 2 two alternating LDS regions; at lengths
+                   // R*2^k: 1 one LDS region shared by the R transforms, 2 two
+                   // such regions used alternately, 0 one region per transform
 };
 template <int L>
 struct Shapes {
@@ -544,26 +553,30 @@ struct Shapes {
 FDAS_SHAPES(256, {64, 8, 0})
 FDAS_SHAPES(512, {64, 8, 0})
 FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
-FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0})
-FDAS_SHAPES(4096, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1}, {128, 8, 1})
-FDAS_SHAPES(8192, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1}, {256, 8, 1})
+FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2})
+FDAS_SHAPES(4096, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1},
+            {128, 8, 1})
+FDAS_SHAPES(8192, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1},
+            {256, 8, 1})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
 // of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144;
-// at 1536 a template row kept for 8 steps measured fastest).
+// at 1536 a template row kept for 8 steps measured fastest).  S = 2 does the
+// same through two alternating regions, one barrier per exchange: 8% faster
+// again at 7168 and 6144, 1% at 5120, slower at 2560.
 // One step per workgroup measured fastest here (the R transforms' values and
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
 FDAS_SHAPES(1536, {128, 8, 0}, {64, 1, 0}, {64, 1, 1})
-FDAS_SHAPES(3072, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
-FDAS_SHAPES(6144, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
-FDAS_SHAPES(12288, {512, 1, 1}, {512, 1, 0}, {256, 1, 0})
+FDAS_SHAPES(3072, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0}, {256, 1, 2})
+FDAS_SHAPES(6144, {256, 1, 2}, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
+FDAS_SHAPES(12288, {512, 1, 1}, {512, 1, 0}, {256, 1, 0}, {512, 1, 2})
 FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
 FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
-FDAS_SHAPES(5120, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
-FDAS_SHAPES(10240, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
-FDAS_SHAPES(7168, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
+FDAS_SHAPES(5120, {128, 1, 2}, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
+FDAS_SHAPES(10240, {256, 1, 1}, {256, 1, 0}, {512, 1, 0}, {256, 1, 2})
+FDAS_SHAPES(7168, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
 #undef FDAS_SHAPES
 
 struct ShapeEntry {
@@ -628,7 +641,7 @@ void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
     const bool ok = sh && v >= 0 && v < c;
     *threads = ok ? sh[v].T : 0;
     *steps = ok ? sh[v].G : 0;
-    if (split) *split = ok && odd_factor(L) == 1 ? sh[v].S : 0;
+    if (split) *split = ok && odd_factor(L) == 1 && sh[v].S == 1;
 }
 
 // Launches of one correlation: step groups [g0, g0 + n) per launch, n chosen so
@@ -653,10 +666,10 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
 {
     using S = typename CStore<P>::T;
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles_t(SPLIT ? L / 2 : L, T, &err);
+    const cplx* tw = fft_twiddles_t(SPLIT == 1 ? L / 2 : L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    if constexpr (SPLIT) {
+    if constexpr (SPLIT == 1) {
         if (!a.bank_split) {
             set_error("split correlation at length %d without a split bank", L);
             return FDAS_ERR_INVALID;
@@ -671,7 +684,7 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
         });
     } else {
         return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
-            hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+            hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE, SPLIT == 2>), grid, dim3(T), 0, s,
                                static_cast<const S*>(a.bank), a.bank_exp, a.Z,
                                static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp,
                                a.H, a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale,
@@ -717,7 +730,7 @@ static int corr_odd_t(const CorrArgs& a, hipStream_t s)
     const double L = (double)R * M;
     const float scale = (float)(1.0 / (L * L));
     return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
-        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float, SH != 0>), grid, dim3(T), 0, s,
+        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float, SH>), grid, dim3(T), 0, s,
                            static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
                            a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
                            static_cast<float*>(a.out), tw, twr, scale, g0);
