@@ -553,7 +553,7 @@ struct Shapes {
 FDAS_SHAPES(256, {64, 8, 0})
 FDAS_SHAPES(512, {64, 8, 0})
 FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
-FDAS_SHAPES(2048, {128, 8, 0}, {256, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2})
+FDAS_SHAPES(2048, {256, 8, 0}, {128, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2})
 FDAS_SHAPES(4096, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1},
             {128, 8, 1})
 FDAS_SHAPES(8192, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1},
@@ -562,20 +562,20 @@ FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
 // of the LDS, so more workgroups per CU: 9-15% faster at 2560, 5120 and 6144;
-// at 1536 a template row kept for 8 steps measured fastest).  S = 2 does the
-// same through two alternating regions, one barrier per exchange: 8% faster
-// again at 7168 and 6144, 1% at 5120, slower at 2560.
+// 1536 fastest with its own region per transform).  S = 2 does the same through
+// two alternating regions, one barrier per exchange: faster again by 10% at
+// 10240, 8% at 7168 and 6144, 6% at 12288, 1-2% at 3072 and 5120; slower at 2560.
 // One step per workgroup measured fastest here (the R transforms' values and
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
-FDAS_SHAPES(1536, {128, 8, 0}, {64, 1, 0}, {64, 1, 1})
-FDAS_SHAPES(3072, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0}, {256, 1, 2})
+FDAS_SHAPES(1536, {64, 1, 0}, {128, 8, 0}, {64, 1, 1})
+FDAS_SHAPES(3072, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
 FDAS_SHAPES(6144, {256, 1, 2}, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
-FDAS_SHAPES(12288, {512, 1, 1}, {512, 1, 0}, {256, 1, 0}, {512, 1, 2})
+FDAS_SHAPES(12288, {512, 1, 2}, {512, 1, 1}, {512, 1, 0}, {256, 1, 0})
 FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
 FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
 FDAS_SHAPES(5120, {128, 1, 2}, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
-FDAS_SHAPES(10240, {256, 1, 1}, {256, 1, 0}, {512, 1, 0}, {256, 1, 2})
+FDAS_SHAPES(10240, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
 FDAS_SHAPES(7168, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
 #undef FDAS_SHAPES
 
