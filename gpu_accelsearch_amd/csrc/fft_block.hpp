@@ -316,40 +316,49 @@ struct BlockFFT {
         passes_shared<NA, 1>(v, lds, tw, t);
     }
 
-    // The same through two LDS regions used alternately (exchange c uses region
-    // c mod 2): one barrier per exchange.  A thread's load of exchange c comes
-    // before its store of exchange c + 1 and so before barrier c + 1, which
-    // every store of exchange c + 2 (the same region) follows.  The leading
-    // barrier orders the first store after the caller's earlier LDS reads.
-    template <int NA, int P>
-    FDAS_HD static void passes_shared2(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    // The same through a ring of NR >= 2 LDS regions: exchange c of a call
+    // uses region c mod NR, with one barrier per exchange.  A thread's load of
+    // exchange c comes before its store of exchange c + 1 and so before barrier
+    // c + 1, which every later store to any region follows; so a store needs no
+    // barrier of its own unless the same thread's previous exchange used the
+    // same region.  Within a call that never happens; across successive calls on
+    // lds it happens only when the last exchange and the first share a region,
+    // and then a leading barrier is issued.  Between calls the caller must not
+    // touch lds.
+    static constexpr int NEXCH = NPASS - 1;
+    template <int NA, int NR>
+    static constexpr bool ring_lead() { return (NEXCH * NA - 1) % NR == 0; }
+    template <int NA, int NR, int P>
+    FDAS_HD static void passes_ring(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
     {
         if constexpr (P < NPASS) {
 #pragma unroll
             for (int a = 0; a < NA; ++a) {
-                cplx* reg = lds + ((((P - 1) * NA + a) & 1) ? LDS_ELEMS : 0);
-                if (P == 1 && a == 0) block_sync();
+                cplx* reg = lds + (((P - 1) * NA + a) % NR) * LDS_ELEMS;
+                if (ring_lead<NA, NR>() && P == 1 && a == 0) block_sync();
                 store<P - 1>(v[a], reg, t);
                 block_sync();
                 load<P>(v[a], reg, t);
             }
 #pragma unroll
             for (int a = 0; a < NA; ++a) compute<P>(v[a], t, tw);
-            passes_shared2<NA, P + 1>(v, lds, tw, t);
+            passes_ring<NA, NR, P + 1>(v, lds, tw, t);
         }
     }
-    // lds holds 2 * LDS_ELEMS.
-    template <int NA>
-    FDAS_HD static void run_shared2(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
+    // lds holds NR * LDS_ELEMS.
+    template <int NA, int NR = 2>
+    FDAS_HD static void run_ring(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
     {
+        static_assert(NR >= 2, "a ring needs two regions");
 #pragma unroll
         for (int a = 0; a < NA; ++a) compute<0>(v[a], t, tw);
-        passes_shared2<NA, 1>(v, lds, tw, t);
+        passes_ring<NA, NR, 1>(v, lds, tw, t);
     }
-    // One transform (as run) through two alternating regions of 2 * LDS_ELEMS.
-    FDAS_HD static void run2(cplx* v, cplx* lds, const cplx* tw, int t)
+    // One transform (as run) through a ring of NR regions of LDS_ELEMS.
+    template <int NR = 2>
+    FDAS_HD static void run_ring1(cplx* v, cplx* lds, const cplx* tw, int t)
     {
-        run_shared2<1>(reinterpret_cast<cplx(*)[E]>(v), lds, tw, t);
+        run_ring<1, NR>(reinterpret_cast<cplx(*)[E]>(v), lds, tw, t);
     }
 
     template <int NA>

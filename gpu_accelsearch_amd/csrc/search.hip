@@ -223,7 +223,8 @@ __device__ __forceinline__ void load_template(const typename CStore<P>::T* __res
 
 // y = IFFT_L(A[k mod L/2] * conj(K[k])), unnormalised.  Position t + i*T needs
 // A[t + (i mod E/2)*T], so each thread loads E/2 segment values.
-template <int L, int T = FftCfgS<L>::T, int P = PREC_F32, bool DB = false>
+// NR: 1 one LDS region; >= 2 a ring of NR regions (BlockFFT::run_ring1).
+template <int L, int T = FftCfgS<L>::T, int P = PREC_F32, int NR = 1>
 __device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
                                           const typename CStore<P>::T* __restrict__ A,
                                           const cplx* __restrict__ tw, cplx* lds, cplx* v, int t)
@@ -235,7 +236,7 @@ __device__ __forceinline__ void corr_ifft(const typename CStore<P>::T* kr,
     for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[t + i * T]);
 #pragma unroll
     for (int i = 0; i < F::E; ++i) v[i] = cmulc(a[i % EH], CStore<P>::get(kr[i]));
-    if constexpr (DB) F::run2(v, lds, tw, t);
+    if constexpr (NR > 1) F::template run_ring1<NR>(v, lds, tw, t);
     else F::run(v, lds, tw, t);
 }
 
@@ -260,9 +261,10 @@ __device__ __forceinline__ bool corr_tile(int Z, int g0, int& z, int& g)
 // Workgroup (z, g) (corr_tile) handles steps [g*G, min(g*G+G, nsteps))
 // with the template row held in registers across them.  Reduced-precision
 // inputs carry per-row / per-segment exponents, folded into the power scale.
-// DB: the transform's exchanges alternate between two LDS regions (one barrier
-// each instead of two).
-template <int L, int G, int T, int P, typename OutE, bool DB>
+// NR >= 2: the transform's exchanges go round a ring of NR LDS regions (one
+// barrier each instead of two; with NR regions such that a step's last exchange
+// and the next step's first use different ones, none between steps either).
+template <int L, int G, int T, int P, typename OutE, int NR>
 __global__ __launch_bounds__(T) void corr_sub_kernel(
     const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
     const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
 {
     using F = BlockFFT<L, T, 1>;
     using S = typename CStore<P>::T;
-    __shared__ cplx lds[(DB ? 2 : 1) * F::LDS_ELEMS];
+    __shared__ cplx lds[NR * F::LDS_ELEMS];
     const int t = threadIdx.x;
     int z, g;
     if (!corr_tile(Z, g0, z, g)) return;
@@ -284,11 +286,11 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
     const int ek = P == PREC_F32 ? 0 : bank_exp[z];
     for (int s = s0; s < s1; ++s) {
         if constexpr (G > 1) {
-            corr_ifft<L, T, P, DB>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T, P, NR>(kr, seg + s * seg_stride + seg_off, tw, lds, v, t);
         } else {
             S k1[F::E];
             load_template<L, T, P>(bank + (size_t)z * L, k1, t);
-            corr_ifft<L, T, P, DB>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
+            corr_ifft<L, T, P, NR>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
         }
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
         OutE* o = out + ((size_t)s * Z + z) * W;
@@ -347,7 +349,7 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
             }
         }
         if constexpr (SH == 1) F::template run_shared<R>(v, lds, tw, t);
-        else if constexpr (SH == 2) F::template run_shared2<R>(v, lds, tw, t);
+        else if constexpr (SH == 2) F::template run_ring<R, 2>(v, lds, tw, t);
         else F::template run_multi<R>(v, lds, tw, t);
         RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
@@ -532,13 +534,15 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // reloads its template row.  Two alternating LDS regions (S = 2) halve the
 // barriers of each exchange: 7% faster at 4096 and 2% at 8192 (512 threads,
 // one workgroup per CU, where every barrier idles the CU), slower with 1024
-// threads.
+// threads.  Three regions (S = 3) would also drop the barrier between steps
+// (3 exchanges per transform), but at 4096 they ran 1.55x slower: 101 KB of
+// LDS leaves one workgroup per CU instead of two.  At 8192 they do not fit.
 namespace {
 struct CorrShape {
     int T, G, S;   // threads, steps per workgroup; S at powers of two: 1 split form
-                   // (two L/2 transforms), 2 two alternating LDS regions; at lengths
-                   // R*2^k: 1 one LDS region shared by the R transforms, 2 two
-                   // such regions used alternately, 0 one region per transform
+                   // (two L/2 transforms), 2 or 3 a ring of S LDS regions; at
+                   // lengths R*2^k: 1 one LDS region shared by the R transforms,
+                   // 2 a ring of two such regions, 0 one region per transform
 };
 template <int L>
 struct Shapes {
@@ -684,7 +688,7 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
         });
     } else {
         return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
-            hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE, SPLIT == 2>), grid, dim3(T), 0, s,
+            hipLaunchKernelGGL((corr_sub_kernel<L, G, T, P, OutE, SPLIT == 0 ? 1 : SPLIT>), grid, dim3(T), 0, s,
                                static_cast<const S*>(a.bank), a.bank_exp, a.Z,
                                static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp,
                                a.H, a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale,
