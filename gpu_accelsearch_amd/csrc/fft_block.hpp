@@ -316,49 +316,55 @@ struct BlockFFT {
         passes_shared<NA, 1>(v, lds, tw, t);
     }
 
-    // The same through a ring of NR >= 2 LDS regions: exchange c of a call
-    // uses region c mod NR, with one barrier per exchange.  A thread's load of
-    // exchange c comes before its store of exchange c + 1 and so before barrier
-    // c + 1, which every later store to any region follows; so a store needs no
+    // The same through a ring of NR >= 2 slots of K LDS regions each: an
+    // exchange moves a group of up to K transforms (all stores, one barrier, all
+    // loads), and exchange c of a call uses slot c mod NR.  A thread's loads of
+    // exchange c come before its stores of exchange c + 1 and so before barrier
+    // c + 1, which every later store to any slot follows; so a store needs no
     // barrier of its own unless the same thread's previous exchange used the
-    // same region.  Within a call that never happens; across successive calls on
-    // lds it happens only when the last exchange and the first share a region,
+    // same slot.  Within a call that never happens; across successive calls on
+    // lds it happens only when the last exchange and the first share a slot,
     // and then a leading barrier is issued.  Between calls the caller must not
     // touch lds.
     static constexpr int NEXCH = NPASS - 1;
-    template <int NA, int NR>
-    static constexpr bool ring_lead() { return (NEXCH * NA - 1) % NR == 0; }
-    template <int NA, int NR, int P>
+    template <int NA, int NR, int K>
+    static constexpr bool ring_lead() { return (NEXCH * ((NA + K - 1) / K) - 1) % NR == 0; }
+    template <int NA, int NR, int K, int P>
     FDAS_HD static void passes_ring(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
     {
         if constexpr (P < NPASS) {
+            constexpr int NG = (NA + K - 1) / K;
 #pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                cplx* reg = lds + (((P - 1) * NA + a) % NR) * LDS_ELEMS;
-                if (ring_lead<NA, NR>() && P == 1 && a == 0) block_sync();
-                store<P - 1>(v[a], reg, t);
+            for (int gi = 0; gi < NG; ++gi) {
+                cplx* slot = lds + (((P - 1) * NG + gi) % NR) * K * LDS_ELEMS;
+                if (ring_lead<NA, NR, K>() && P == 1 && gi == 0) block_sync();
+#pragma unroll
+                for (int b = 0; b < K; ++b)
+                    if (gi * K + b < NA) store<P - 1>(v[gi * K + b], slot + b * LDS_ELEMS, t);
                 block_sync();
-                load<P>(v[a], reg, t);
+#pragma unroll
+                for (int b = 0; b < K; ++b)
+                    if (gi * K + b < NA) load<P>(v[gi * K + b], slot + b * LDS_ELEMS, t);
             }
 #pragma unroll
             for (int a = 0; a < NA; ++a) compute<P>(v[a], t, tw);
-            passes_ring<NA, NR, P + 1>(v, lds, tw, t);
+            passes_ring<NA, NR, K, P + 1>(v, lds, tw, t);
         }
     }
-    // lds holds NR * LDS_ELEMS.
-    template <int NA, int NR = 2>
+    // lds holds NR * K * LDS_ELEMS.
+    template <int NA, int NR = 2, int K = 1>
     FDAS_HD static void run_ring(cplx (*v)[E], cplx* lds, const cplx* tw, int t)
     {
-        static_assert(NR >= 2, "a ring needs two regions");
+        static_assert(NR >= 2 && K >= 1 && K <= NA, "a ring needs two slots of 1..NA regions");
 #pragma unroll
         for (int a = 0; a < NA; ++a) compute<0>(v[a], t, tw);
-        passes_ring<NA, NR, 1>(v, lds, tw, t);
+        passes_ring<NA, NR, K, 1>(v, lds, tw, t);
     }
     // One transform (as run) through a ring of NR regions of LDS_ELEMS.
     template <int NR = 2>
     FDAS_HD static void run_ring1(cplx* v, cplx* lds, const cplx* tw, int t)
     {
-        run_ring<1, NR>(reinterpret_cast<cplx(*)[E]>(v), lds, tw, t);
+        run_ring<1, NR, 1>(reinterpret_cast<cplx(*)[E]>(v), lds, tw, t);
     }
 
     template <int NA>

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
             }
         }
         if constexpr (SH == 1) F::template run_shared<R>(v, lds, tw, t);
-        else if constexpr (SH == 2) F::template run_ring<R, 2>(v, lds, tw, t);
+        else if constexpr (SH >= 2) F::template run_ring<R, 2, SH / 2>(v, lds, tw, t);
         else F::template run_multi<R>(v, lds, tw, t);
         RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
@@ -540,9 +540,10 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 namespace {
 struct CorrShape {
     int T, G, S;   // threads, steps per workgroup; S at powers of two: 1 split form
-                   // (two L/2 transforms), 2 or 3 a ring of S LDS regions; at
-                   // lengths R*2^k: 1 one LDS region shared by the R transforms,
-                   // 2 a ring of two such regions, 0 one region per transform
+                   // (two L/2 transforms), 2 a ring of two LDS regions; at lengths
+                   // R*2^k: 0 one region per transform, 1 one region shared by the
+                   // R transforms, 2K a ring of two slots of K regions (K
+                   // transforms exchanged per barrier)
 };
 template <int L>
 struct Shapes {
@@ -569,6 +570,9 @@ FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // 1536 fastest with its own region per transform).  S = 2 does the same through
 // two alternating regions, one barrier per exchange: faster again by 10% at
 // 10240, 8% at 7168 and 6144, 6% at 12288, 1-2% at 3072 and 5120; slower at 2560.
+// S = 2K (K > 1) exchanges K transforms per barrier through 2K regions: 3%
+// faster at 7168 with K = 2, but 1.2-1.4x slower at 5120 and 6144 (the LDS
+// cuts the workgroups per CU), and slower with K = 3 or 4.
 // One step per workgroup measured fastest here (the R transforms' values and
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
@@ -580,7 +584,7 @@ FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
 FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
 FDAS_SHAPES(5120, {128, 1, 2}, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
 FDAS_SHAPES(10240, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
-FDAS_SHAPES(7168, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
+FDAS_SHAPES(7168, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0}, {256, 1, 4})
 #undef FDAS_SHAPES
 
 struct ShapeEntry {
