@@ -330,7 +330,8 @@ def main() -> int:
         "candidates": int(len(cands)),
         "autotune": None if tune is None else
         {"corr": {str(k): {"variant": v["variant"], "threads": v["threads"],
-                           "steps_per_wg": v["steps_per_wg"], "split": v["split"]}
+                           "steps_per_wg": v["steps_per_wg"], "split": v["split"],
+                           "lds_form": v["lds_form"]}
                   for k, v in tune["corr"].items()},
          "harmsum": {k: tune["harmsum"][k] for k in ("threads", "rows", "cols")}},
         "fft_lengths": [h[2] for h in s.harmonics],

@@ -131,7 +131,7 @@ class FdasTuneReport(C.Structure):
                 ("corr_us", (C.c_double * TUNE_MAXV) * TUNE_MAX),
                 ("harmsum_variant", C.c_int), ("harmsum_threads", C.c_int),
                 ("harmsum_rows", C.c_int), ("harmsum_cols", C.c_int),
-                ("harmsum_ms", C.c_double * TUNE_MAXV)]
+                ("harmsum_ms", C.c_double * TUNE_MAXV), ("corr_form", C.c_int * TUNE_MAX)]
 
 
 vp = C.c_void_p

@@ -454,6 +454,7 @@ def autotune(params: SearchParams, fftlens=()) -> dict:
                     "ms_per_trial": [m for m in rep.harmsum_ms[:lib().fdas_harmsum_variant_count()]]},
         "corr": {rep.corr_len[i]: {"variant": rep.corr_variant[i], "threads": rep.corr_threads[i],
                                    "steps_per_wg": rep.corr_steps[i], "split": bool(rep.corr_split[i]),
+                                   "lds_form": rep.corr_form[i],
                                    "us_per_variant": [u for u in rep.corr_us[i]
                                                       [:lib().fdas_corr_variant_count(rep.corr_len[i])]]}
                  for i in range(rep.ncorr)},

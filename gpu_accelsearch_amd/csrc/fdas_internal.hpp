@@ -189,7 +189,8 @@ int launch_corr(int L, const CorrArgs& a, hipStream_t s, int variant = -1);
 int corr_variant_count(int L);
 int corr_variant(int L);
 int set_corr_variant(int L, int v);
-void corr_variant_shape(int L, int v, int* threads, int* steps, int* split = nullptr);
+void corr_variant_shape(int L, int v, int* threads, int* steps, int* split = nullptr,
+                        int* form = nullptr);
 // Split-form rows of a full FP32 bank (Z rows of length L).
 int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t s);
 // Reduced-precision copy of a bank of Z rows of length L.

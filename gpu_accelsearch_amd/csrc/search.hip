@@ -642,7 +642,7 @@ int set_corr_variant(int L, int v)
     return FDAS_OK;
 }
 
-void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
+void corr_variant_shape(int L, int v, int* threads, int* steps, int* split, int* form)
 {
     int c = 0;
     const CorrShape* sh = shapes_of(L, &c);
@@ -650,6 +650,7 @@ void corr_variant_shape(int L, int v, int* threads, int* steps, int* split)
     *threads = ok ? sh[v].T : 0;
     *steps = ok ? sh[v].G : 0;
     if (split) *split = ok && odd_factor(L) == 1 && sh[v].S == 1;
+    if (form) *form = ok ? sh[v].S : 0;
 }
 
 // Launches of one correlation: step groups [g0, g0 + n) per launch, n chosen so

@@ -298,7 +298,7 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
         }
         rep->corr_variant[k] = corr_variant(L);
         corr_variant_shape(L, rep->corr_variant[k], &rep->corr_threads[k], &rep->corr_steps[k],
-                           &rep->corr_split[k]);
+                           &rep->corr_split[k], &rep->corr_form[k]);
     }
     // 2. harmonic-sum tile shape: whole trials at the first valid candidate length
     for (size_t i = 0; rc == FDAS_OK && i < cand.size() && base->numharm > 1; ++i) {

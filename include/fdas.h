@@ -376,6 +376,10 @@ typedef struct fdas_tune_report {
     int harmsum_variant;                     /* selected harmonic-sum tile shape   */
     int harmsum_threads, harmsum_rows, harmsum_cols;  /* cols 0: 896 or 768, by row width */
     double harmsum_ms[FDAS_TUNE_MAXV];       /* ms per trial per shape (0: n/a)    */
+    /* LDS exchange form of corr_variant: 0 one region per transform; 1 the split
+     * form (powers of two) or one region shared by the R transforms (lengths
+     * R*2^k); 2K a ring of two slots of K regions, one barrier per exchange */
+    int corr_form[FDAS_TUNE_MAX];
 } fdas_tune_report;
 
 int fdas_autotune(const fdas_params *base, const int *fftlens, int nfftlen,

@@ -71,6 +71,9 @@ def test_autotune_selects_and_applies(fd):
         assert fd.lib().fdas_get_corr_variant(l) == c["variant"]
         if len(us) > 1:   # the shape in use stays unless another is 3% faster
             assert 0.97 * us[c["variant"]] <= min(us)
+        # split form only at powers of two, and reported as LDS form 1 there
+        assert c["lds_form"] in (0, 1, 2, 4)
+        assert c["split"] == (c["lds_form"] == 1 and l & (l - 1) == 0)
     with fd.AccelSearch(p) as s:
         assert s.info.fftlen[0] == rep["fftlen"]
 
