@@ -532,7 +532,8 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 // than 8 at 4096 and 8192) but leave fewer workgroups for short trials.  At
 // 16384 the registers are needed for the transform itself, so every step
 // reloads its template row.  Two alternating LDS regions (S = 2) halve the
-// barriers of each exchange: 7% faster at 4096 and 2% at 8192 (512 threads,
+// barriers of each exchange: 7% faster at 4096 (and 3% more with 8 steps per
+// workgroup instead of 16) and 2% at 8192 (512 threads,
 // one workgroup per CU, where every barrier idles the CU), slower with 1024
 // threads.  Three regions (S = 3) would also drop the barrier between steps
 // (3 exchanges per transform), but at 4096 they ran 1.55x slower: 101 KB of
@@ -559,10 +560,10 @@ FDAS_SHAPES(256, {64, 8, 0})
 FDAS_SHAPES(512, {64, 8, 0})
 FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
 FDAS_SHAPES(2048, {256, 8, 0}, {128, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2})
-FDAS_SHAPES(4096, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0}, {256, 8, 1},
-            {128, 8, 1})
+FDAS_SHAPES(4096, {512, 8, 2}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0},
+            {256, 8, 1}, {128, 8, 1})
 FDAS_SHAPES(8192, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1},
-            {256, 8, 1})
+            {256, 8, 1}, {512, 8, 2})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
