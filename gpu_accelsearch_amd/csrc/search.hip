@@ -578,7 +578,7 @@ FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
 FDAS_SHAPES(1536, {64, 1, 0}, {128, 8, 0}, {64, 1, 1})
-FDAS_SHAPES(3072, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
+FDAS_SHAPES(3072, {128, 1, 2}, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0}, {256, 8, 0})
 FDAS_SHAPES(6144, {256, 1, 2}, {256, 1, 1}, {512, 1, 1}, {256, 1, 0}, {512, 1, 0}, {512, 8, 0})
 FDAS_SHAPES(12288, {512, 1, 2}, {512, 1, 1}, {512, 1, 0}, {256, 1, 0})
 FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
