@@ -525,17 +525,17 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 }
 
 // Launch shapes of the fused correlation per FFT length: (threads, steps per
-// workgroup).  Variant 0 is the default; the autotuner may select another per
-// length.  More threads give more waves per SIMD and fewer values per thread
-// (a smaller radix, so one more LDS pass).  More steps per workgroup load the
-// template row and the hoisted twiddles fewer times (16 measured 3-5% faster
-// than 8 at 4096 and 8192) but leave fewer workgroups for short trials.  At
-// 16384 the registers are needed for the transform itself, so every step
-// reloads its template row.  Two alternating LDS regions (S = 2) halve the
-// barriers of each exchange: 7% faster at 4096 (and 3% more with 8 steps per
-// workgroup instead of 16) and 2% at 8192 (512 threads,
-// one workgroup per CU, where every barrier idles the CU), slower with 1024
-// threads.  Three regions (S = 3) would also drop the barrier between steps
+// workgroup, LDS form).  Variant 0 is the default; the autotuner may select
+// another per length.  More threads give more waves per SIMD and fewer values
+// per thread (a smaller radix, so one more LDS pass).  More steps per workgroup
+// load the template row and the hoisted twiddles fewer times (with one LDS
+// region 16 measured 3-5% faster than 8 at 4096 and 8192) but leave fewer
+// workgroups for short trials.  At 16384 the registers are needed for the
+// transform itself, so every step reloads its template row.  Two alternating
+// LDS regions (S = 2) halve the barriers of each exchange: 7% faster at 4096
+// and 2% at 8192 (512 threads, one workgroup per CU, where every barrier idles
+// the CU), slower with 1024 threads; with them 8 steps per workgroup beat 16 by
+// 3% at 4096.  Three regions (S = 3) would also drop the barrier between steps
 // (3 exchanges per transform), but at 4096 they ran 1.55x slower: 101 KB of
 // LDS leaves one workgroup per CU instead of two.  At 8192 they do not fit.
 namespace {
@@ -573,7 +573,9 @@ FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // 10240, 8% at 7168 and 6144, 6% at 12288, 1-2% at 3072 and 5120; slower at 2560.
 // S = 2K (K > 1) exchanges K transforms per barrier through 2K regions: 3%
 // faster at 7168 with K = 2, but 1.2-1.4x slower at 5120 and 6144 (the LDS
-// cuts the workgroups per CU), and slower with K = 3 or 4.
+// cuts the workgroups per CU), and slower with K = 3 or 4.  At 3072, 128
+// threads (8 values per transform) beat 256 by 5%; at 6144 and 12288 fewer
+// threads than the defaults were 13-20% slower.
 // One step per workgroup measured fastest here (the R transforms' values and
 // twiddles already take the registers a cached template row would need).
 FDAS_SHAPES(768, {64, 1, 1}, {64, 1, 0}, {64, 8, 0})
