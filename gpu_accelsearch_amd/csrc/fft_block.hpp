@@ -45,6 +45,10 @@ namespace fdas {
 
 #ifdef __HIPCC__
 __device__ __forceinline__ void block_sync() { __syncthreads(); }
+#elif defined(FDAS_HOST_BARRIER)
+// A host test that runs each workgroup thread as an OS thread supplies the barrier.
+void host_barrier();
+inline void block_sync() { host_barrier(); }
 #else
 inline void block_sync() {}
 #endif

@@ -1,0 +1,139 @@
+// Race check of the LDS exchange ring in gpu_accelsearch_amd/csrc/fft_block.hpp.
+// Each workgroup thread runs as an OS thread and block_sync() is a real barrier.
+// After every barrier thread 0 sleeps before going on, so the other threads run
+// ahead to their next barrier: any LDS region that is stored again before a
+// barrier separates it from thread 0's pending loads hands thread 0 overwritten
+// values, and its outputs stop matching the DFT.  Successive calls on the same
+// LDS (as a workgroup's successive steps) check the barrier between calls.
+// Built by tests/test_lds_ring.py with g++ and called through ctypes.
+#define FDAS_HOST_BARRIER 1
+#include "../gpu_accelsearch_amd/csrc/fft_block.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace fdas {
+
+namespace {
+std::mutex g_mu;
+std::condition_variable g_cv;
+int g_count = 0, g_total = 0;
+long g_gen = 0;
+int g_sleep_us = 0;
+thread_local int g_tid = -1;
+}  // namespace
+
+void host_barrier()
+{
+    {
+        std::unique_lock<std::mutex> lk(g_mu);
+        const long gen = g_gen;
+        if (++g_count == g_total) {
+            g_count = 0;
+            ++g_gen;
+            g_cv.notify_all();
+        } else {
+            g_cv.wait(lk, [&] { return g_gen != gen; });
+        }
+    }
+    if (g_tid == 0) std::this_thread::sleep_for(std::chrono::microseconds(g_sleep_us));
+}
+
+}  // namespace fdas
+
+using namespace fdas;
+
+// The ring's exchanges with the leading barrier left out: a planted mistake, so
+// the test can show that the check sees a missing barrier.
+template <typename F, int NA, int NR, int P>
+static void passes_ring_nolead(cplx (*v)[F::E], cplx* lds, const cplx* tw, int t)
+{
+    if constexpr (P < F::NPASS) {
+        for (int a = 0; a < NA; ++a) {
+            cplx* reg = lds + (((P - 1) * NA + a) % NR) * F::LDS_ELEMS;
+            F::template store<P - 1>(v[a], reg, t);
+            block_sync();
+            F::template load<P>(v[a], reg, t);
+        }
+        for (int a = 0; a < NA; ++a) F::template compute<P>(v[a], t, tw);
+        passes_ring_nolead<F, NA, NR, P + 1>(v, lds, tw, t);
+    }
+}
+
+// calls x NA transforms of length N (sign DIR) by T threads through a ring of
+// NR slots of K regions; in/out [calls][NA][N] interleaved complex.
+template <int N, int T, int DIR, int NA, int NR, int K, bool PLANT>
+static void ring_run(int calls, const float* in, float* out)
+{
+    using F = BlockFFT<N, T, DIR>;
+    constexpr int E = F::E;
+    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds((size_t)NR * K * F::LDS_ELEMS);
+    F::build_twiddles(tw.data());
+    g_total = T;
+    g_count = 0;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            g_tid = t;
+            for (int c = 0; c < calls; ++c) {
+                cplx v[NA][E];
+                for (int a = 0; a < NA; ++a)
+                    for (int i = 0; i < E; ++i) {
+                        const size_t n = ((size_t)c * NA + a) * N + F::in_index(t, i);
+                        v[a][i] = mk(in[2 * n], in[2 * n + 1]);
+                    }
+                if constexpr (PLANT) {
+                    for (int a = 0; a < NA; ++a) F::template compute<0>(v[a], t, tw.data());
+                    passes_ring_nolead<F, NA, NR, 1>(v, lds.data(), tw.data(), t);
+                } else {
+                    F::template run_ring<NA, NR, K>(v, lds.data(), tw.data(), t);
+                }
+                for (int a = 0; a < NA; ++a)
+                    for (int i = 0; i < E; ++i) {
+                        const size_t n = ((size_t)c * NA + a) * N + F::out_index(t, i);
+                        out[2 * n] = v[a][i].x;
+                        out[2 * n + 1] = v[a][i].y;
+                    }
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+// Returns -1 for a shape not listed; lead (if not null) gets whether the ring
+// issues the leading barrier for that shape.
+extern "C" int ring_race(int n, int t, int dir, int na, int nr, int k, int plant, int calls,
+                         int sleep_us, const float* in, float* out, int* lead)
+{
+    g_sleep_us = sleep_us;
+#define CASE(NN, TT, NA, NR, KK)                                                                 \
+    if (n == NN && t == TT && na == NA && nr == NR && k == KK) {                                  \
+        if (lead) *lead = BlockFFT<NN, TT, 1>::ring_lead<NA, NR, KK>();                           \
+        if (plant) {                                                                              \
+            if (dir < 0) ring_run<NN, TT, -1, NA, NR, KK, true>(calls, in, out);                  \
+            else ring_run<NN, TT, +1, NA, NR, KK, true>(calls, in, out);                          \
+        } else {                                                                                  \
+            if (dir < 0) ring_run<NN, TT, -1, NA, NR, KK, false>(calls, in, out);                 \
+            else ring_run<NN, TT, +1, NA, NR, KK, false>(calls, in, out);                         \
+        }                                                                                         \
+        return 0;                                                                                 \
+    }
+    // the library's shapes: one transform at powers of two, R transforms of
+    // M = L/R at the lengths R * 2^k (K transforms exchanged per barrier)
+    CASE(2048, 256, 1, 2, 1)
+    CASE(4096, 512, 1, 2, 1)
+    CASE(8192, 512, 1, 2, 1)
+    CASE(1024, 128, 3, 2, 1)
+    CASE(1024, 256, 3, 2, 1)
+    CASE(2048, 256, 3, 2, 1)
+    CASE(1024, 128, 5, 2, 1)
+    CASE(2048, 256, 5, 2, 1)
+    CASE(1024, 256, 7, 2, 1)
+    CASE(1024, 128, 7, 2, 1)
+    CASE(1024, 256, 7, 2, 2)
+    CASE(4096, 512, 3, 2, 1)
+#undef CASE
+    return -1;
+}
