@@ -63,14 +63,17 @@ static void passes_ring_nolead(cplx (*v)[F::E], cplx* lds, const cplx* tw, int t
     }
 }
 
-// calls x NA transforms of length N (sign DIR) by T threads through a ring of
-// NR slots of K regions; in/out [calls][NA][N] interleaved complex.
-template <int N, int T, int DIR, int NA, int NR, int K, bool PLANT>
+// calls x NA transforms of length N (sign DIR) by T threads; in/out
+// [calls][NA][N] interleaved complex.  FORM 2: a ring of NR slots of K regions
+// (run_ring); 0: a region per transform (run, run_multi); 1: one region shared
+// by the NA transforms (run_shared).
+template <int N, int T, int DIR, int NA, int NR, int K, bool PLANT, int FORM = 2>
 static void ring_run(int calls, const float* in, float* out)
 {
     using F = BlockFFT<N, T, DIR>;
     constexpr int E = F::E;
-    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds((size_t)NR * K * F::LDS_ELEMS);
+    constexpr int NREG = FORM == 2 ? NR * K : FORM == 1 ? 1 : NA;
+    std::vector<cplx> tw(F::TW_COUNT > 0 ? F::TW_COUNT : 1), lds((size_t)NREG * F::LDS_ELEMS);
     F::build_twiddles(tw.data());
     g_total = T;
     g_count = 0;
@@ -88,6 +91,12 @@ static void ring_run(int calls, const float* in, float* out)
                 if constexpr (PLANT) {
                     for (int a = 0; a < NA; ++a) F::template compute<0>(v[a], t, tw.data());
                     passes_ring_nolead<F, NA, NR, 1>(v, lds.data(), tw.data(), t);
+                } else if constexpr (FORM == 1) {
+                    F::template run_shared<NA>(v, lds.data(), tw.data(), t);
+                } else if constexpr (FORM == 0 && NA == 1) {
+                    F::run(v[0], lds.data(), tw.data(), t);
+                } else if constexpr (FORM == 0) {
+                    F::template run_multi<NA>(v, lds.data(), tw.data(), t);
                 } else {
                     F::template run_ring<NA, NR, K>(v, lds.data(), tw.data(), t);
                 }
@@ -134,6 +143,30 @@ extern "C" int ring_race(int n, int t, int dir, int na, int nr, int k, int plant
     CASE(1024, 128, 7, 2, 1)
     CASE(1024, 256, 7, 2, 2)
     CASE(4096, 512, 3, 2, 1)
+#undef CASE
+    return -1;
+}
+
+// The other exchange forms (form 0: a region per transform, 1: one shared
+// region), two barriers per exchange, at the shapes the library uses them.
+extern "C" int form_race(int n, int t, int dir, int na, int form, int calls, int sleep_us,
+                         const float* in, float* out)
+{
+    g_sleep_us = sleep_us;
+#define CASE(NN, TT, NA, FF)                                                                      \
+    if (n == NN && t == TT && na == NA && form == FF) {                                           \
+        if (dir < 0) ring_run<NN, TT, -1, NA, 2, 1, false, FF>(calls, in, out);                   \
+        else ring_run<NN, TT, +1, NA, 2, 1, false, FF>(calls, in, out);                           \
+        return 0;                                                                                 \
+    }
+    CASE(1024, 64, 1, 0)
+    CASE(8192, 512, 1, 0)
+    CASE(16384, 512, 1, 0)
+    CASE(256, 64, 3, 1)
+    CASE(256, 64, 5, 1)
+    CASE(512, 128, 5, 1)
+    CASE(512, 64, 3, 0)
+    CASE(512, 128, 7, 0)
 #undef CASE
     return -1;
 }

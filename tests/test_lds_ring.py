@@ -62,3 +62,25 @@ def test_ring_check_sees_a_missing_barrier(ring, n, t, na):
     err, lead = run(ring, n, t, na, 2, 1, 1, 1, 2, 7)
     assert lead == 1
     assert err.max() > 1e-2, err
+
+
+# (length, threads, transforms, form): one region (1024, 8192, 16384), one region
+# shared by R transforms (768, 1280, 2560), a region per transform (1536, and the
+# seven lock-step transforms of the 7168 segment preparation)
+FORMS = [(1024, 64, 1, 0), (8192, 512, 1, 0), (16384, 512, 1, 0), (256, 64, 3, 1), (256, 64, 5, 1),
+         (512, 128, 5, 1), (512, 64, 3, 0), (512, 128, 7, 0)]
+
+
+@pytest.mark.parametrize("n,t,na,form", FORMS)
+def test_other_exchange_forms_have_no_race(ring, n, t, na, form):
+    ring.form_race.argtypes = [C.c_int] * 7 + [C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(n + t + na)
+    calls = 3
+    for sign in (1, -1):
+        x = (rng.standard_normal((calls, na, n)) + 1j * rng.standard_normal((calls, na, n))).astype(np.complex64)
+        out = np.zeros_like(x)
+        assert ring.form_race(n, t, sign, na, form, calls, SLEEP_US, x.ctypes.data, out.ctypes.data) == 0
+        xd = x.astype(np.complex128)
+        ref = np.fft.fft(xd, axis=-1) if sign < 0 else np.fft.ifft(xd, axis=-1) * n
+        err = np.abs(out - ref).max() / np.sqrt(np.mean(np.abs(ref) ** 2))
+        assert err < 1e-6, (sign, err)
