@@ -572,7 +572,7 @@ FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // two alternating regions, one barrier per exchange: faster again by 10% at
 // 10240, 8% at 7168 and 6144, 6% at 12288, 1-2% at 3072 and 5120; slower at 2560.
 // S = 2K (K > 1) exchanges K transforms per barrier through 2K regions: 3%
-// faster at 7168 with K = 2, but 1.2-1.4x slower at 5120 and 6144 (the LDS
+// faster at 7168 with K = 2 (the default there), but 1.2-1.4x slower at 5120 and 6144 (the LDS
 // cuts the workgroups per CU), and slower with K = 3 or 4.  At 3072, 128
 // threads (8 values per transform) beat 256 by 5%; at 6144 and 12288 fewer
 // threads than the defaults were 13-20% slower.
@@ -587,7 +587,7 @@ FDAS_SHAPES(1280, {64, 1, 1}, {64, 1, 0})
 FDAS_SHAPES(2560, {128, 1, 1}, {128, 1, 0}, {64, 1, 0})
 FDAS_SHAPES(5120, {128, 1, 2}, {128, 1, 1}, {256, 1, 1}, {256, 1, 0}, {128, 1, 0})
 FDAS_SHAPES(10240, {256, 1, 2}, {256, 1, 1}, {256, 1, 0}, {512, 1, 0})
-FDAS_SHAPES(7168, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0}, {256, 1, 4})
+FDAS_SHAPES(7168, {256, 1, 4}, {256, 1, 2}, {128, 1, 2}, {256, 1, 1}, {128, 1, 1}, {256, 1, 0})
 #undef FDAS_SHAPES
 
 struct ShapeEntry {
