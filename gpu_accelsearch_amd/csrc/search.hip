@@ -319,6 +319,7 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
     using F = BlockFFT<M, T, 1>;
     using S = typename CStore<P>::T;
     constexpr int E = F::E, EH = E / 2, MH = M / 2;
+    static_assert(SH <= 1 || (SH % 2 == 0 && SH / 2 <= R), "LDS form: 0, 1 or 2K with K <= R");
     __shared__ cplx lds[(SH == 0 ? R : SH) * F::LDS_ELEMS];
     const int t = threadIdx.x;
     int z, g;
