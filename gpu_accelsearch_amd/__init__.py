@@ -7,7 +7,8 @@ first call that needs it does, and raises if it has not been built.
 from ._native import FdasError, last_error, lib  # noqa: F401
 from .accel import (ACCELCAND_DTYPE, AccelSearch, BatchSearch, autotune, candidate_sigma,
                     batch_from_files, load_dm_trials, numindep, power_for_sigma, readData,
-                    read_inf, resolve_fftlen, harmonic_fftlens, sift_raw, CANDIDATE_DTYPE, DeviceArray,
+                    read_inf, resolve_fftlen, harmonic_fftlens, corr_variant_shapes, sift_raw,
+                    CANDIDATE_DTYPE, DeviceArray,
                     SearchParams,  # noqa: F401
                     Server, StagePipeline, SubharmonicBank, calc_required_r, calc_required_z,
                     calculateFFTLength, complexToComplexFFT, createSubharmonicInfo, deredden,
@@ -22,6 +23,7 @@ __all__ = [
     "calc_required_r", "index_from_r", "index_from_z", "getTotalHarmonics",
     "z_response_halfwidth", "calculateFFTLength", "stage_powcut", "synchronize", "lib",
     "FdasError", "last_error", "CANDIDATE_DTYPE", "autotune", "resolve_fftlen", "harmonic_fftlens",
+    "corr_variant_shapes",
     "ACCELCAND_DTYPE", "candidate_sigma", "power_for_sigma", "numindep", "sift_raw",
     "readData", "read_inf", "load_dm_trials", "batch_from_files",
 ]

@@ -213,6 +213,7 @@ PROTOTYPES = [
     ("fdas_set_harmsum_variant", i32, [i32]),
     ("fdas_get_corr_variant", i32, [i32]),
     ("fdas_set_corr_variant", i32, [i32, i32]),
+    ("fdas_corr_variant_shape", i32, [i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     ("fdas_resolve_fftlen", i32, [C.POINTER(FdasParams)]),
     ("fdas_harmonic_fftlens", i32, [C.POINTER(FdasParams), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("fdas_server_create", i32, [C.POINTER(FdasServerConfig), C.POINTER(vp)]),

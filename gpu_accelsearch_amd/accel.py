@@ -461,6 +461,20 @@ def autotune(params: SearchParams, fftlens=()) -> dict:
     }
 
 
+def corr_variant_shapes(fftlen: int) -> list:
+    """The correlation launch shapes selectable at one FFT length, variant 0 first:
+    threads, steps per workgroup, LDS exchange form, and whether it is the split
+    form (two L/2 transforms; powers of two only)."""
+    out = []
+    for v in range(lib().fdas_corr_variant_count(fftlen)):
+        t, g, f = C.c_int(), C.c_int(), C.c_int()
+        check(lib().fdas_corr_variant_shape(fftlen, v, C.byref(t), C.byref(g), C.byref(f)),
+              "fdas_corr_variant_shape")
+        out.append({"threads": t.value, "steps_per_wg": g.value, "lds_form": f.value,
+                    "split": f.value == 1 and fftlen & (fftlen - 1) == 0})
+    return out
+
+
 def resolve_fftlen(params: SearchParams) -> int:
     cp = params.to_c()
     return lib().fdas_resolve_fftlen(C.byref(cp))

@@ -242,6 +242,21 @@ extern "C" int fdas_get_corr_variant(int fftlen) { return corr_variant(fftlen); 
 
 extern "C" int fdas_set_corr_variant(int fftlen, int variant) { return set_corr_variant(fftlen, variant); }
 
+extern "C" int fdas_corr_variant_shape(int fftlen, int variant, int* threads, int* steps_per_wg,
+                                       int* lds_form)
+{
+    if (!threads || !steps_per_wg || !lds_form) {
+        set_error("fdas_corr_variant_shape: null argument");
+        return FDAS_ERR_INVALID;
+    }
+    if (variant < 0 || variant >= corr_variant_count(fftlen)) {
+        set_error("correlation variant %d out of range at length %d", variant, fftlen);
+        return FDAS_ERR_INVALID;
+    }
+    corr_variant_shape(fftlen, variant, threads, steps_per_wg, nullptr, lds_form);
+    return FDAS_OK;
+}
+
 extern "C" int fdas_resolve_fftlen(const fdas_params* params)
 {
     if (!params) {

@@ -390,6 +390,10 @@ int fdas_get_harmsum_variant(void);
 int fdas_set_harmsum_variant(int variant);
 int fdas_get_corr_variant(int fftlen);
 int fdas_set_corr_variant(int fftlen, int variant);
+/* Shape of a correlation variant: threads, steps per workgroup and LDS exchange
+ * form (as fdas_tune_report.corr_form; 1 at a power of two is the split form). */
+int fdas_corr_variant_shape(int fftlen, int variant, int *threads, int *steps_per_wg,
+                            int *lds_form);
 /* The fundamental FFT length a plan with these parameters would use. */
 int fdas_resolve_fftlen(const fdas_params *params);
 

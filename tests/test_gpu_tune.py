@@ -1,5 +1,6 @@
 """Autotuner: every correlation launch shape agrees with the oracle (1e-4 of
-max(P_ref, 1)), and the tuner's choices apply to later plans."""
+max(P_ref, 1)) and, in reduced-precision storage, with the default shape; the
+tuner's choices apply to later plans."""
 import numpy as np
 import pytest
 
@@ -106,3 +107,55 @@ def test_every_harmsum_shape_is_bit_identical(fd):
                     assert key[1] == ref[1]
     finally:
         L.fdas_set_harmsum_variant(default)
+
+
+# oracle tolerances of the reduced-precision storage modes (as the precision tests)
+PREC_TOL = {"fp16": 2e-3, "bf16": 2e-2, "i16": 1e-3}
+
+
+@pytest.mark.parametrize("precision", ["bf16", "fp16", "i16"])
+@pytest.mark.parametrize("fftlen", [8192, 4096])
+def test_every_corr_variant_reduced_precision(fd, oracle, precision, fftlen):
+    """In reduced-precision storage every correlation shape reads the same stored
+    template rows and segments and computes in FP32, so its planes equal the
+    default shape's within FP32 rounding (2e-5 of max(P, 1)).  The split form
+    stores its own packed bank (sums and differences of template values), so it
+    is held to the mode's oracle tolerance instead, as the default shape is."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 20
+    x = synth.plain_series(n, seed=22, r0=30000.3, z0=100, power=80.0)
+    ref = oracle.Search(x, n, 6.4e-5, 200, 8, fftlen)
+    L = fd.lib()
+    try:
+        P = fd.SearchParams(n=n, zmax=200, numharm=8, fftlen=fftlen, precision=precision)
+        with fd.AccelSearch(P) as s:
+            lens = sorted({hm[2] for hm in s.harmonics})
+            shapes = {l: fd.corr_variant_shapes(l) for l in lens}
+            nv = max(len(v) for v in shapes.values())
+            steps = [0, s.nsteps // 2]
+            want = [ref.step_plane(st) for st in steps]
+            s.prepare(x)
+            for l in lens:
+                L.fdas_set_corr_variant(l, 0)
+            base = [s.run_steps(st, 1)[0].copy() for st in steps]
+            for b, w in zip(base, want):
+                assert float(np.max(np.abs(b - w) / np.maximum(w, 1.0))) <= PREC_TOL[precision]
+            for v in range(1, nv):
+                split = False
+                for l in lens:
+                    vl = min(v, len(shapes[l]) - 1)
+                    split |= shapes[l][vl]["split"]
+                    assert L.fdas_set_corr_variant(l, vl) == 0
+                print_case("corr variant", v, precision, "lengths", lens, "split" if split else "")
+                for st, b, w in zip(steps, base, want):
+                    got = s.run_steps(st, 1)[0]
+                    if split:
+                        e = float(np.max(np.abs(got - w) / np.maximum(w, 1.0)))
+                        assert e <= PREC_TOL[precision], (v, st, e)
+                    else:
+                        e = float(np.max(np.abs(got - b) / np.maximum(b, 1.0)))
+                        assert e <= 2e-5, (v, st, e)
+    finally:
+        for l in ALL_LENGTHS:
+            L.fdas_set_corr_variant(l, 0)
+        ref.close()

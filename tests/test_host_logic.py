@@ -229,3 +229,32 @@ def test_sub_harmonic_lengths_fit_every_step(oracle, zmax, numharm, fftlen, n):
             L = short[h]
             assert g[h]["numdata"] <= L // 2, (k, h, g[h]["numdata"], L)
             assert g[h]["numrs"] + 4 * g[h]["binoffset"] <= L, (k, h, g[h]["numrs"], g[h]["binoffset"], L)
+
+
+def test_corr_variant_shapes_table():
+    """Every correlation length's launch shapes (host tables, no GPU): at most
+    FDAS_TUNE_MAXV per length, threads a power of two dividing the transform
+    length with at least 2 values each, the split form only at powers of two,
+    and an LDS ring of 2K regions only for K <= R transforms; out-of-range
+    variants are rejected."""
+    import ctypes as C
+    import gpu_accelsearch_amd as fd
+    from gpu_accelsearch_amd import _native as N
+    lens = (256, 512, 1024, 2048, 4096, 8192, 16384, 768, 1536, 3072, 6144, 12288, 1280, 2560,
+            5120, 10240, 7168)
+    for L in lens:
+        shapes = fd.corr_variant_shapes(L)
+        assert 1 <= len(shapes) <= N.TUNE_MAXV, L
+        r = next((q for q in (3, 5, 7) if L % q == 0), 1)
+        m = L // r
+        for sh in shapes:
+            t, form = sh["threads"], sh["lds_form"]
+            assert t & (t - 1) == 0 and m % t == 0 and m // t >= 2, (L, sh)
+            if r == 1:
+                assert form in (0, 1, 2), (L, sh)
+                assert sh["split"] == (form == 1)
+            else:
+                assert not sh["split"] and (form in (0, 1) or (form % 2 == 0 and form // 2 <= r)), (L, sh)
+        z = C.c_int()
+        assert fd.lib().fdas_corr_variant_shape(L, len(shapes), C.byref(z), C.byref(z), C.byref(z)) != 0
+    assert fd.corr_variant_shapes(777) == []
