@@ -199,3 +199,38 @@ def test_batch_from_presto_files(fd, tmp_path):
             b2.upload(t, x)
         b2.run()
         assert got == _cand_set(b2.candidates()) and len(got) > 0
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_graph_replay_production_shape(fd, precision):
+    """Graph replay at the production shape (zmax 200, 8 harmonics, fftlen 8192:
+    sub-harmonics at 7168, 6144, 5120, 4096, 2560 and 1280, the two-region LDS
+    exchange) reproduces the eager candidates bit for bit on every replay, with
+    the inputs changed between replays."""
+    from gpu_accelsearch_amd import synth
+    lengths = [1 << 21, (1 << 21) - 6]
+    xs = [synth.plain_series(n, seed=60 + t, r0=70000.3 + 900.0 * t, z0=-70.0 + 50.0 * t, power=100.0)
+          for t, n in enumerate(lengths)]
+    ys = [synth.plain_series(n, seed=80 + t, r0=90000.7 - 700.0 * t, z0=60.0 - 30.0 * t, power=100.0)
+          for t, n in enumerate(lengths)]
+    P = fd.SearchParams(n=0, zmax=200, numharm=8, fftlen=8192, precision=precision,
+                        powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+    print_case("production-shape eager", precision)
+    with fd.BatchSearch(P, lengths) as e:
+        assert list(e.info.fftlen[:8]) == [8192, 4096, 2560, 6144, 1280, 4096, 5120, 7168]
+        want = []
+        for data in (xs, ys):
+            for t, x in enumerate(data):
+                e.upload(t, x)
+            e.run()
+            want.append(_cand_set(e.candidates()))
+            assert not e.overflowed
+    assert want[0] != want[1] and len(want[0]) > 0 and len(want[1]) > 0
+    print_case("production-shape graph replay", precision)
+    with fd.BatchSearch(P, lengths, graph=True) as g:
+        for rep, k in enumerate((0, 0, 1, 0, 1)):
+            for t, x in enumerate((xs, ys)[k]):
+                g.upload(t, x)
+            g.run()
+            assert _cand_set(g.candidates()) == want[k], rep
+            g.synchronize()
