@@ -30,7 +30,8 @@ namespace fdas {
     constexpr int HS_CPT = HS_CB / HS_T;                                       \
     constexpr int HS_MAXR = HS_RJ + 1;                                         \
     constexpr int HS_MAXQ = ((HS_CB + 8) / 4 + HS_T - 1) / HS_T;               \
-    static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");
+    static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");  \
+    static_assert(HS_CB % HS_T == 0 && HS_RJ * HS_CPT <= 32, "a hit bit per cell");
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
 // fundamental powers on entry and, when store is set, the summed powers on exit
