@@ -227,6 +227,9 @@ def main() -> int:
                     help="storage of spectra segments and templates (arithmetic is FP32)")
     ap.add_argument("--batch-trials", type=int, default=8,
                     help="DM trials of the batched reduced-precision form (0: skip)")
+    ap.add_argument("--batch-fftlen", type=int, default=0,
+                    help="fftlen of the batched reduced-precision form (0: autotuned among "
+                         "4096, 8192 and 16384)")
     ap.add_argument("--stage-steps", type=int, default=12,
                     help="steps sampled for the reference-structured stage form (0: skip)")
     ap.add_argument("--batch-workload-trials", type=int, default=256,
@@ -342,8 +345,11 @@ def main() -> int:
     if args.stage_steps > 0:
         out["stage_form_fp32"] = stage_form(spec, n, zmax, numharm, fftlen, info, args.stage_steps)
     if args.batch_trials > 0:
-        rp = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=0, precision="bf16")
-        bt = fd.autotune(rp, fftlens=(4096, 8192, 16384))
+        if args.batch_fftlen > 0:
+            bt = {"fftlen": args.batch_fftlen, "ms_per_trial": None}
+        else:
+            rp = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=0, precision="bf16")
+            bt = fd.autotune(rp, fftlens=(4096, 8192, 16384))
         out["batch_bf16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "bf16", bt["fftlen"])
         out["batch_bf16"]["fftlen_ms_per_trial"] = bt["ms_per_trial"]
         out["batch_i16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "i16", bt["fftlen"])
