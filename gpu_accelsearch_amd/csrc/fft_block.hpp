@@ -186,12 +186,18 @@ struct BlockFFT {
         return o;
     }
     static constexpr int TW_COUNT = twoff(NPASS);
-    // LDS slot of exchange position i: one slot of padding per 32.  A conflict-free
-    // XOR swizzle of the low 4 bits (bits LE..LE+7 folded in) was measured: it
-    // removed every bank conflict but made the 4096-point, 8-step shape 50% slower
-    // (more address arithmetic in a kernel at its register limit), and left the
-    // others within 4%; the correlation kernels are VALU-bound.
-    FDAS_HD static int pad(int i) { return i + (i >> 5); }
+    // LDS slot of exchange position i.  At 4 values per thread: the low 4 bits XORed
+    // with bits 2..5 and 6..9, a unit-triangular map over GF(2), so a bijection on
+    // each aligned 16-slot block; a 16-lane group of 8-byte accesses then covers
+    // all 32 banks once in every store and load of every pass.  Otherwise one slot
+    // of padding per 32: the same swizzle at 8 values per thread made the
+    // 4096-point, 8-step shape 50% slower (more address arithmetic in a kernel at
+    // its register limit), and the correlation kernels are VALU-bound.
+    FDAS_HD static int pad(int i)
+    {
+        if constexpr (E == 4) return i ^ (((i >> 2) ^ (i >> 6)) & 15);
+        else return i + (i >> 5);
+    }
 
     template <int P>
     FDAS_HD static void compute(cplx* v, int t, const cplx* tw)
