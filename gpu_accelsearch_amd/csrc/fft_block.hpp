@@ -193,11 +193,22 @@ struct BlockFFT {
     // of padding per 32: the same swizzle at 8 values per thread made the
     // 4096-point, 8-step shape 50% slower (more address arithmetic in a kernel at
     // its register limit), and the correlation kernels are VALU-bound.
-    FDAS_HD static int pad(int i)
+    FDAS_HD static constexpr int pad(int i)
     {
         if constexpr (E == 4) return i ^ (((i >> 2) ^ (i >> 6)) & 15);
         else return i + (i >> 5);
     }
+    static constexpr bool pad_is_injective()
+    {
+        bool seen[LDS_ELEMS] = {};
+        for (int i = 0; i < N; ++i) {
+            const int p = pad(i);
+            if (p < 0 || p >= LDS_ELEMS || seen[p]) return false;
+            seen[p] = true;
+        }
+        return true;
+    }
+    static_assert(pad_is_injective(), "exchange slot map must be one-to-one within the region");
 
     template <int P>
     FDAS_HD static void compute(cplx* v, int t, const cplx* tw)
