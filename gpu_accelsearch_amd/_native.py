@@ -156,6 +156,7 @@ PROTOTYPES = [
     ("complexToComplexFFT", C.c_uint, [vp, C.c_uint, C.POINTER(vp), i32]),
     ("inverseComplexToComplexFFT", C.c_uint, [vp, C.c_uint, C.POINTER(vp), i32]),
     ("deredden", None, [vp, C.c_uint]),
+    ("fdas_deredden_noalloc", i32, [vp, C.c_uint, vp]),
     ("get_fourier_amplitudes", vp, [i32, i32, vp, C.c_long, C.c_long]),
     ("spreadData", None, [vp, i32, i32, C.c_uint, vp]),
     ("generate_harmonics_ffdot", None, [u16, u16, C.POINTER(SubharmonicInfo), C.c_double,

@@ -7,8 +7,10 @@
 // block's powers, (2) one wavefront per segment: interpolated normalisation with
 // the reference's float/double types and its fast inverse square root.
 #include <cmath>
+#include <mutex>
 
 #include "fdas_internal.hpp"
+#include "whiten_runs.hpp"
 
 namespace fdas {
 
@@ -198,4 +200,151 @@ int whiten(cplx* d_spec, const WhitenSchedule& ws, hipStream_t s)
     return check_launch("whiten_norm_kernel");
 }
 
+// ------------------------------------------------------------------ allocation-free form
+// The block schedule as a run table (whiten_runs.hpp) passed by value in the kernel
+// arguments, so whitening needs no device tables: only the block medians, which
+// live in the file-scope array `scratch`.
+namespace {
+constexpr unsigned kNoallocMaxBins = 1u << 26;   // 663 030 median blocks
+constexpr int kNoallocMaxBlocks = 1 << 20;       // 4 MiB of medians
+constexpr int kMaxDevices = 64;
+
+__global__ __launch_bounds__(256) void whiten_median_runs_kernel(const cplx* __restrict__ spec,
+                                                                 const WhitenRuns w,
+                                                                 float* __restrict__ median)
+{
+    __shared__ float buf[4][kMaxBlock];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int b = blockIdx.x * 4 + wv;
+    const bool valid = b < w.nblocks;   // uniform per wavefront
+    int s0 = 0, n = 0;
+    if (valid) whiten_run_block(w, b, s0, n);
+    for (int i = lane; i < n; i += 64) buf[wv][i] = power_of(spec[s0 + i]);
+    __syncthreads();
+    if (valid) {
+        const float m = wave_select(buf[wv], n, (n - 1) / 2, lane);
+        if (lane == 0) median[b] = m;
+    }
+}
+
+// Segment j of whiten_norm_kernel with its bounds and block lengths from the runs:
+// segment 0 is [1, 1 + len_0/2), segment j < nblocks is [hi_{j-1}, start_j + len_j/2),
+// and the tail segment nblocks ends at nbins.
+__global__ __launch_bounds__(256) void whiten_norm_runs_kernel(cplx* __restrict__ spec,
+                                                               const WhitenRuns w,
+                                                               const float* __restrict__ median)
+{
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int j = blockIdx.x * 4 + wv;
+    const int nblocks = w.nblocks;
+    if (j > nblocks) return;
+    int sj = 0, lj = 0, sp = 0, lp = 0;
+    if (j < nblocks) whiten_run_block(w, j, sj, lj);
+    if (j > 0) whiten_run_block(w, j - 1, sp, lp);
+    const int a = j == 0 ? 1 : sp + lp / 2;
+    const int e = j == nblocks ? (int)w.nbins : sj + lj / 2;
+    float mean_old, dslope;
+    bool flat = false;
+    if (j == 0) {
+        mean_old = (float)((double)median[0] / kLn2);
+        dslope = 0.0f;
+        flat = true;
+        if (lane == 0) spec[0] = mk(1.0f, 0.0f);
+    } else {
+        const int jj = (j < nblocks) ? j : nblocks - 1;   // block whose slope applies
+        if (jj == 0) {
+            mean_old = (float)((double)median[0] / kLn2);
+            dslope = 1.0f;
+        } else {
+            int sq = 0, lq = 0, sr = 0, lr = 0;
+            whiten_run_block(w, jj - 1, sq, lq);
+            whiten_run_block(w, jj, sr, lr);
+            const float prev = (float)((double)median[jj - 1] / kLn2);
+            const double mean_new = (double)median[jj] / kLn2;
+            dslope = (float)((mean_new - (double)prev) / (0.5 * (double)(lq + lr)));
+            mean_old = (j < nblocks) ? prev : (float)mean_new;
+        }
+    }
+    const float norm_flat = quake_rsqrt(mean_old);
+    for (int b = a + lane; b < e; b += 64) {
+        float norm;
+        if (flat) {
+            norm = norm_flat;
+        } else {
+            const float t = dslope * (float)(b - a);
+            norm = quake_rsqrt(mean_old + t);
+        }
+        cplx v = spec[b];
+        v.x = v.x * norm;
+        v.y = v.y * norm;
+        spec[b] = v;
+    }
+}
+
+// Block medians of the allocation-free form.  Each GPU holds its own copy for the
+// life of the process; every call on a GPU overwrites it (all medians a call reads
+// are written by that call's median kernel first, so nothing needs clearing), and
+// calls that overlap on one GPU race on it.
+static __device__ __align__(256) float scratch[kNoallocMaxBlocks];
+
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this GPU's `scratch` (looked up once per device; the host-side
+// name is only a handle, never a device address).
+int scratch_address(float** out)
+{
+    int dev = 0;
+    FDAS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("fdas_deredden_noalloc: device %d beyond the %d supported", dev, kMaxDevices);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+        g_scratch_ptr[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return FDAS_OK;
+}
+}  // namespace
+
 }  // namespace fdas
+
+using namespace fdas;
+
+extern "C" int fdas_deredden_noalloc(fdas_complex* d_fft, unsigned numOfBins, void* hip_stream)
+{
+    if (!d_fft) {
+        set_error("fdas_deredden_noalloc: null spectrum");
+        return FDAS_ERR_INVALID;
+    }
+    if (numOfBins < 16 || numOfBins > kNoallocMaxBins) {
+        set_error("fdas_deredden_noalloc: %u bins outside [16, %u]", numOfBins, kNoallocMaxBins);
+        return FDAS_ERR_INVALID;
+    }
+    WhitenRuns w;
+    if (whiten_runs(numOfBins, &w) != 0) {
+        set_error("fdas_deredden_noalloc: more than %d block runs at %u bins", kWhitenMaxRuns, numOfBins);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    if (w.nblocks > kNoallocMaxBlocks) {
+        set_error("fdas_deredden_noalloc: %d median blocks exceed the %d of the scratch array",
+                  w.nblocks, kNoallocMaxBlocks);
+        return FDAS_ERR_UNSUPPORTED;
+    }
+    float* med = nullptr;
+    FDAS_TRY(scratch_address(&med));
+    hipStream_t s = static_cast<hipStream_t>(hip_stream);
+    cplx* spec = reinterpret_cast<cplx*>(d_fft);
+    hipLaunchKernelGGL(whiten_median_runs_kernel, dim3((w.nblocks + 3) / 4), dim3(256), 0, s, spec, w,
+                       med);
+    FDAS_TRY(check_launch("whiten_median_runs_kernel"));
+    hipLaunchKernelGGL(whiten_norm_runs_kernel, dim3((w.nblocks + 1 + 3) / 4), dim3(256), 0, s, spec,
+                       w, static_cast<const float*>(med));
+    return check_launch("whiten_norm_runs_kernel");
+}

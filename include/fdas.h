@@ -95,6 +95,15 @@ unsigned inverseComplexToComplexFFT(fdas_complex *device_idata, unsigned int inp
 
 /* headers/bpd_noise.h:16 -- whitening of host bins [0, numOfBins), in place. */
 void deredden(fdas_complex *fft, unsigned numOfBins);
+/* No counterpart in the reference: deredden on DEVICE bins [0, numOfBins), in
+ * place, for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  It calls no allocator: its block medians live
+ * in a static device array of 4 MiB per GPU (held until the process ends), which
+ * limits numOfBins to [16, 2^26].  Calls that overlap on one GPU race on that
+ * array.  Queued on hip_stream (a hipStream_t; NULL: the default stream).
+ * Returns FDAS_OK, or FDAS_ERR_INVALID for a null pointer or a size outside the
+ * limit.  Results are bit-identical to deredden's. */
+int fdas_deredden_noalloc(fdas_complex *d_fft, unsigned numOfBins, void *hip_stream);
 
 /* headers/bpd_search.cuh:34-35 -- host data movement helpers. */
 fdas_complex *get_fourier_amplitudes(int loBin, int numbins, fdas_complex *fft,
