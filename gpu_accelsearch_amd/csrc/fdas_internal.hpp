@@ -197,16 +197,18 @@ int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t
 int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
                      hipStream_t s);
 
-// Harmonic sum over d_plane[step][Z0][U] (fundamental powers on entry), stage
-// thresholds and candidate emission; the summed powers are written back only
-// when store_plane is set.
+// Harmonic sum over d_fund[step][Z0][U] (fundamental powers) and the sub-harmonic
+// planes d_sub, both in the plane element of storage mode prec (FP32, or P16 in the
+// reduced-precision modes); stage thresholds and candidate emission.  The summed
+// FP32 powers go to d_out[step][Z0][U] when d_out is not null (with FP32 planes it
+// may equal d_fund).
 int harmsum_variant_count();
 int harmsum_variant();
 int set_harmsum_variant(int v);
 void harmsum_variant_shape(int v, int* threads, int* rows, int* cols);
-int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
-                   int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
-                   int* d_ncand, bool store_plane, hipStream_t s);
+int launch_harmsum(const void* d_fund, float* d_out, const void* d_sub, int prec, const HarmSumArgs& hs,
+                   int step0, int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
+                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s);
 
 // Plan creation on a given stream (0: the plan makes its own) with buffers from
 // al (nullptr: hipMalloc).  Implemented in plan.hip.

@@ -33,12 +33,36 @@ namespace fdas {
     static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");  \
     static_assert(HS_CB % HS_T == 0 && HS_RJ * HS_CPT <= 32, "a hit bit per cell");
 
-// grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); plane[s][Z0][U] holds the
-// fundamental powers on entry and, when store is set, the summed powers on exit
-// (every cell is summed and thresholded either way).
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
-__global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane,
-                                                       const float* __restrict__ sub,
+// Four consecutive plane elements as one load (16 bytes of FP32, 8 of P16), and
+// their FP32 values.
+template <typename PE>
+struct Quad;
+template <>
+struct Quad<float> {
+    using V = float4;
+    __device__ __forceinline__ static V zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
+    __device__ __forceinline__ static float4 values(V v) { return v; }
+};
+template <>
+struct Quad<unsigned short> {
+    using V = uint2;
+    __device__ __forceinline__ static V zero() { return make_uint2(0u, 0u); }
+    __device__ __forceinline__ static float4 values(V v)
+    {
+        using P = PStore<unsigned short>;
+        return make_float4(P::get((unsigned short)(v.x & 0xffffu)), P::get((unsigned short)(v.x >> 16)),
+                           P::get((unsigned short)(v.y & 0xffffu)), P::get((unsigned short)(v.y >> 16)));
+    }
+};
+
+// grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); fund[s][Z0][U] holds the
+// fundamental powers (plane element PE: FP32 or P16) and, when store is set, the
+// summed FP32 powers go to out[s][Z0][U] (every cell is summed and thresholded
+// either way).  With FP32 planes out may be fund itself: each cell is read and
+// written by the same thread.
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename PE>
+__global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* out,
+                                                       const PE* __restrict__ sub,
                                                        HarmSumArgs hs, int step0, int Z0,
                                                        const int* __restrict__ fullrlo,
                                                        const int* __restrict__ rlo_h,
@@ -55,26 +79,29 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     const int ncol = min(HS_CB, U - c0);
     const unsigned q0 = 2u * (unsigned)fullrlo[gstep];
     const int* rl = rlo_h + (size_t)gstep * MAX_HARM;
-    float* base_row = plane + ((size_t)s * Z0 + j0) * U;
+    const PE* base_row = fund + ((size_t)s * Z0 + j0) * U;
 
-    // fundamental powers; their threshold test waits until the first round's
-    // loads are in flight too, so both arrive behind one memory wait
+    // fundamental powers, loaded from clamped (always valid) cells without a
+    // branch so that every load is in flight at once; they are decoded and
+    // tested once the first round's loads are in flight too, so both arrive
+    // behind one memory wait (cells outside the tile are never stored or tested)
     float acc[HS_RJ][HS_CPT], p0[HS_RJ][HS_CPT];
+    PE raw[HS_RJ][HS_CPT];
     unsigned hit = 0;
     bool first = true;
 #pragma unroll
     for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
         for (int k = 0; k < HS_CPT; ++k) {
-            const int col = c0 + t + k * HS_T;
-            const bool ok = r < nrow && col < c0 + ncol;
-            acc[r][k] = ok ? base_row[(size_t)r * U + col] : 0.f;
+            const int col = min(c0 + t + k * HS_T, c0 + ncol - 1);
+            raw[r][k] = base_row[(size_t)min(r, nrow - 1) * U + col];
         }
     auto mark_fundamental = [&]() {
 #pragma unroll
         for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
+                acc[r][k] = PStore<PE>::get(raw[r][k]);
                 p0[r][k] = acc[r][k];
                 const int col = c0 + t + k * HS_T;
                 if (r < nrow && col < c0 + ncol && acc[r][k] > hs.powcut[0])
@@ -137,15 +164,16 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
         for (int g = h; g < h2; ++g) {
             const int nca = u(geo[g][1]), nr = u(geo[g][2]), base = u(geo[g][6 + HS_RJ]);
             const int nq = nca >> 2;
-            const float4* src = reinterpret_cast<const float4*>(sub + gbase[g]);
+            using Q = Quad<PE>;
+            const typename Q::V* src = reinterpret_cast<const typename Q::V*>(sub + gbase[g]);
             const int wq = hs.W[g] >> 2;
-            float4 v[HS_MAXR][HS_MAXQ];
+            typename Q::V v[HS_MAXR][HS_MAXQ];
 #pragma unroll
             for (int rr = 0; rr < HS_MAXR; ++rr)
 #pragma unroll
                 for (int k = 0; k < HS_MAXQ; ++k) {
                     const int q = t + k * HS_T;
-                    v[rr][k] = (rr < nr && q < nq) ? src[(long long)rr * wq + q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                    v[rr][k] = (rr < nr && q < nq) ? src[(long long)rr * wq + q] : Q::zero();
                 }
             if (first) {
                 mark_fundamental();
@@ -157,7 +185,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
 #pragma unroll
                 for (int k = 0; k < HS_MAXQ; ++k) {
                     const int q = t + k * HS_T;
-                    if (rr < nr && q < nq) dst[rr * nq + q] = v[rr][k];
+                    if (rr < nr && q < nq) dst[rr * nq + q] = Q::values(v[rr][k]);
                 }
         }
         __syncthreads();
@@ -190,12 +218,13 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     }
 
     if (store) {
+        float* out_row = out + ((size_t)s * Z0 + j0) * U;
 #pragma unroll
         for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
                 const int col = c0 + t + k * HS_T;
-                if (r < nrow && col < c0 + ncol) base_row[(size_t)r * U + col] = acc[r][k];
+                if (r < nrow && col < c0 + ncol) out_row[(size_t)r * U + col] = acc[r][k];
             }
     }
 
@@ -218,7 +247,7 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
                 const int c = hs.c[h], lm = hs.lm[h];
                 const int rind = hs_rind(q0 + col, c, lm, 2 * rl[h]);
                 const int zind = hs_zind(zz, c, lm, hs.zlo[h]);
-                a2 += sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind];
+                a2 += PStore<PE>::get(sub[hs.sub_off[h] + s * hs.sub_stride[h] + (long long)zind * hs.W[h] + rind]);
                 if (((h + 1) & h) == 0) {
                     ++st;
                     if (a2 > hs.powcut[st]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, st, a2);
@@ -228,10 +257,10 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(float* __restrict__ plane
     }
 }
 
-template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
-static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0,
-                     int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
-                     fdas_candidate* d_cands, int* d_ncand, int store, hipStream_t s)
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS, typename PE>
+static int harmsum_pe(const void* d_fund, float* d_out, const void* d_sub, const HarmSumArgs& hs,
+                      int step0, int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
+                      fdas_candidate* d_cands, int* d_ncand, int store, hipStream_t s)
 {
     for (int h = 1; h < hs.H; ++h) {
         if ((hs.W[h] & 3) || (hs.sub_off[h] & 3) || (hs.sub_stride[h] & 3)) {
@@ -245,9 +274,25 @@ static int harmsum_t(float* d_plane, const float* d_sub, const HarmSumArgs& hs, 
     }
     dim3 grid((unsigned)((hs.U + HS_CB - 1) / HS_CB), (unsigned)((Z0 + HS_RJ - 1) / HS_RJ),
               (unsigned)nsteps);
-    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS>), grid, dim3(HS_T), 0, s,
-                       d_plane, d_sub, hs, step0, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, store);
+    hipLaunchKernelGGL((harmsum_kernel<HS_T, HS_RJ, HS_CB, HS_LDS, PE>), grid, dim3(HS_T), 0, s,
+                       static_cast<const PE*>(d_fund), d_out, static_cast<const PE*>(d_sub), hs, step0,
+                       Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, store);
     return check_launch("harmsum_kernel");
+}
+
+template <int HS_T, int HS_RJ, int HS_CB, int HS_LDS>
+static int harmsum_t(const void* d_fund, float* d_out, const void* d_sub, int prec,
+                     const HarmSumArgs& hs, int step0, int nsteps, int Z0, const int* d_fullrlo,
+                     const int* d_rlo_h, fdas_candidate* d_cands, int* d_ncand, int store,
+                     hipStream_t s)
+{
+    if (prec == PREC_F32)
+        return harmsum_pe<HS_T, HS_RJ, HS_CB, HS_LDS, float>(d_fund, d_out, d_sub, hs, step0, nsteps,
+                                                             Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand,
+                                                             store, s);
+    return harmsum_pe<HS_T, HS_RJ, HS_CB, HS_LDS, unsigned short>(d_fund, d_out, d_sub, hs, step0,
+                                                                  nsteps, Z0, d_fullrlo, d_rlo_h,
+                                                                  d_cands, d_ncand, store, s);
 }
 
 // Tile shapes (threads, rows, columns, staged floats).  The default, variant 0,
@@ -290,13 +335,13 @@ void harmsum_variant_shape(int v, int* threads, int* rows, int* cols)
     *cols = ok ? kHsShapes[v].CB : 0;
 }
 
-int launch_harmsum(float* d_plane, const float* d_sub, const HarmSumArgs& hs, int step0, int nsteps,
-                   int Z0, const int* d_fullrlo, const int* d_rlo_h, fdas_candidate* d_cands,
-                   int* d_ncand, bool store_plane, hipStream_t s)
+int launch_harmsum(const void* d_fund, float* d_out, const void* d_sub, int prec, const HarmSumArgs& hs,
+                   int step0, int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
+                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
 {
     if (nsteps <= 0) return FDAS_OK;
-    const int st = store_plane ? 1 : 0;
-#define HS_ARGS d_plane, d_sub, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, st, s
+    const int st = d_out ? 1 : 0;
+#define HS_ARGS d_fund, d_out, d_sub, prec, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, st, s
     switch (g_hs_variant.load()) {
         case 1: return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
         case 2: return harmsum_t<128, 4, 896, 4608>(HS_ARGS);

@@ -174,8 +174,9 @@ struct fdas_plan {
     cplx* d_r2c_work = nullptr;
     void* d_seg = nullptr;
     int* d_seg_exp = nullptr;
-    float* d_sub = nullptr;
-    float* d_plane = nullptr;
+    void* d_sub = nullptr;          // sub-harmonic power planes (plane element of prec)
+    void* d_plane = nullptr;        // fundamental power plane (same element)
+    size_t pbytes = sizeof(float);  // plane element: FP32, or P16 in reduced precision
     fdas_candidate* d_cands = nullptr;
     int* d_ncand = nullptr;
     int cand_cap = 0;
@@ -269,6 +270,7 @@ int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
     FDAS_TRY(validate(p));
     pl->prec = p.precision;
     pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : 4;
+    pl->pbytes = p.precision == FDAS_PREC_FP32 ? sizeof(float) : sizeof(unsigned short);
     pl->uselen = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
     if (pl->uselen <= 0 || (pl->uselen & 1) || pl->uselen >= 65536 ||
         h_fft_length(pl->uselen, 1, 1, p.zmax) != p.fftlen) {
@@ -466,8 +468,8 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     const int U = pl->uselen, H = pl->H;
 
     // chunking: per-item buffer bytes against a device-memory budget
-    const size_t per_item = (size_t)pl->seg_stride * pl->cbytes + (size_t)pl->sub_total * sizeof(float) +
-                            (size_t)pl->Z0 * U * sizeof(float) + (size_t)H * sizeof(int);
+    const size_t per_item = (size_t)pl->seg_stride * pl->cbytes + (size_t)pl->sub_total * pl->pbytes +
+                            (size_t)pl->Z0 * U * pl->pbytes + (size_t)H * sizeof(int);
     // default: as many items per chunk as 24 GiB of chunk buffers hold (a whole
     // 2^23-sample trial at zmax 200: fewer, larger launches measured fastest, and
     // overlapping the harmonic sum of one chunk with the correlations of the
@@ -516,8 +518,8 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     FDAS_TRY(plan_alloc(pl, &pl->d_seg, (size_t)pl->chunk * pl->seg_stride * pl->cbytes));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_seg_exp, (size_t)pl->chunk * H * sizeof(int)));
     if (pl->sub_total > 0)
-        FDAS_TRY(plan_alloc(pl, (void**)&pl->d_sub, (size_t)pl->chunk * pl->sub_total * sizeof(float)));
-    FDAS_TRY(plan_alloc(pl, (void**)&pl->d_plane, (size_t)pl->chunk * pl->Z0 * U * sizeof(float)));
+        FDAS_TRY(plan_alloc(pl, &pl->d_sub, (size_t)pl->chunk * pl->sub_total * pl->pbytes));
+    FDAS_TRY(plan_alloc(pl, &pl->d_plane, (size_t)pl->chunk * pl->Z0 * U * pl->pbytes));
     pl->cand_cap = p.max_candidates > 0 ? p.max_candidates : (1 << 20);
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_cands, (size_t)pl->cand_cap * sizeof(fdas_candidate)));
     FDAS_TRY(plan_alloc(pl, (void**)&pl->d_ncand, sizeof(int)));
@@ -611,10 +613,13 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         };
         for (int h = 1; h < pl->H; ++h) {
             TimedLaunch tl(pl, FDAS_K_CORR_SUB);
-            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, pl->d_sub + pl->hs.sub_off[h], pl->W[h]),
-                                 pl->stream));
+            char* sub_h = static_cast<char*>(pl->d_sub) + (size_t)pl->hs.sub_off[h] * pl->pbytes;
+            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, sub_h, pl->W[h]), pl->stream));
         }
-        float* plane = d_planes ? d_planes + (size_t)(c0 - item0) * pl->Z0 * U : pl->d_plane;
+        // summed powers for the caller, FP32; with FP32 planes the fundamental
+        // plane is that buffer itself and the sum goes back in place
+        float* out = d_planes ? d_planes + (size_t)(c0 - item0) * pl->Z0 * U : nullptr;
+        void* plane = out && pl->prec == FDAS_PREC_FP32 ? static_cast<void*>(out) : pl->d_plane;
         {
             TimedLaunch tl(pl, FDAS_K_CORR_FUND);
             FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream));
@@ -625,8 +630,8 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         // plane is stored only into a caller's buffer: the internal plane's sole
         // consumer is the threshold, which the kernel applies to every cell.
         TimedLaunch tl(pl, FDAS_K_HARMSUM);
-        FDAS_TRY(launch_harmsum(plane, pl->d_sub, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo, pl->d_rlo_h,
-                                pl->d_cands, pl->d_ncand, d_planes != nullptr, pl->stream));
+        FDAS_TRY(launch_harmsum(plane, out, pl->d_sub, pl->prec, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo,
+                                pl->d_rlo_h, pl->d_cands, pl->d_ncand, pl->stream));
     }
     return FDAS_OK;
 }

@@ -6,11 +6,14 @@
 // power-of-two exponent per template row
 // or per segment, chosen so that the largest component maps below 2^15.  Scaling
 // by a power of two is exact, so the exponents are folded into the power scale
-// (|y|^2 scales with 2^(2 (e_template + e_segment))).  All FFT arithmetic, power
-// planes and power accumulation stay in FP32.
+// (|y|^2 scales with 2^(2 (e_template + e_segment))).  The reduced-precision modes
+// also store the power planes in 16 bits (P16 below).  All FFT arithmetic and power
+// accumulation stay in FP32.
 #pragma once
 
 #include <hip/hip_fp16.h>
+
+#include <type_traits>
 
 #include "fdas_common.hpp"
 
@@ -88,7 +91,14 @@ struct CStore<PREC_I16> {
     }
 };
 
-// Power-plane element (FP32 in every mode).
+// Power-plane element: FP32 with FP32 storage; in the reduced-precision modes a
+// 16-bit unsigned float ("P16": 6 exponent bits, 10 mantissa bits, no sign, since
+// powers are non-negative) covering [2^-32, 2^32) with a fixed exponent bias, so no
+// per-row exponent is needed.  A code c stands for the float whose bits are
+// (c << 13) + (95 << 23): the top 16 bits of an FP32 value whose exponent is
+// offset by 95 (2^-32).  Encoding rounds to nearest even (relative error at most
+// 2^-11), sends values below 2^-32 to code 0 (read back as 2^-32) and saturates at
+// 2^32 (1 - 2^-11).  Harmonic sums and thresholds stay FP32.
 template <typename E>
 struct PStore;
 template <>
@@ -96,6 +106,25 @@ struct PStore<float> {
     __device__ __forceinline__ static float get(float v) { return v; }
     __device__ __forceinline__ static float put(float v) { return v; }
 };
+template <>
+struct PStore<unsigned short> {
+    static constexpr unsigned kBias = 95u;
+    __device__ __forceinline__ static float get(unsigned short c)
+    {
+        return __uint_as_float(((unsigned)c << 13) + (kBias << 23));
+    }
+    __device__ __forceinline__ static unsigned short put(float v)
+    {
+        const unsigned u = __float_as_uint(v);
+        const int q = (int)((u + 0xfffu + ((u >> 13) & 1u)) >> 13) - (int)(kBias << 10);
+        return (unsigned short)min(max(q, 0), 65535);
+    }
+};
+
+// Plane element of storage mode P.
+template <int P>
+using PlaneT = typename std::conditional<P == PREC_F32, float, unsigned short>::type;
+inline size_t plane_bytes(int prec) { return prec == PREC_F32 ? sizeof(float) : sizeof(unsigned short); }
 
 // Exponent e with max_abs * 2^-e < 2^15 (0 for an all-zero block).
 __device__ __forceinline__ int block_exponent(float max_abs)

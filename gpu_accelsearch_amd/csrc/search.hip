@@ -707,14 +707,15 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
     }
 }
 
+// Power planes are FP32 with FP32 storage and P16 in the reduced-precision modes.
 template <int L, int T, int G, int SPLIT>
 static int corr_p(const CorrArgs& a, hipStream_t s)
 {
     switch (a.prec) {
-        case PREC_F16: return corr_t<L, T, G, SPLIT, PREC_F16, float>(a, s);
-        case PREC_BF16: return corr_t<L, T, G, SPLIT, PREC_BF16, float>(a, s);
-        case PREC_I16: return corr_t<L, T, G, SPLIT, PREC_I16, float>(a, s);
-        default: return corr_t<L, T, G, SPLIT, PREC_F32, float>(a, s);
+        case PREC_F16: return corr_t<L, T, G, SPLIT, PREC_F16, PlaneT<PREC_F16>>(a, s);
+        case PREC_BF16: return corr_t<L, T, G, SPLIT, PREC_BF16, PlaneT<PREC_BF16>>(a, s);
+        case PREC_I16: return corr_t<L, T, G, SPLIT, PREC_I16, PlaneT<PREC_I16>>(a, s);
+        default: return corr_t<L, T, G, SPLIT, PREC_F32, PlaneT<PREC_F32>>(a, s);
     }
 }
 
@@ -743,10 +744,10 @@ static int corr_odd_t(const CorrArgs& a, hipStream_t s)
     const double L = (double)R * M;
     const float scale = (float)(1.0 / (L * L));
     return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
-        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, float, SH>), grid, dim3(T), 0, s,
+        hipLaunchKernelGGL((corr_odd_kernel<R, M, G, T, P, PlaneT<P>, SH>), grid, dim3(T), 0, s,
                            static_cast<const S*>(a.bank), a.bank_exp, a.Z, static_cast<const S*>(a.seg),
                            a.seg_stride, a.seg_off, a.seg_exp, a.H, a.h, a.nsteps, a.off, a.W,
-                           static_cast<float*>(a.out), tw, twr, scale, g0);
+                           static_cast<PlaneT<P>*>(a.out), tw, twr, scale, g0);
         return check_launch("corr_odd_kernel");
     });
 }

@@ -124,9 +124,12 @@ def test_template_banks_shared(fd):
         b.close()
 
 
-def test_batch_workload_numharm16_autotuned(fd, oracle):
+@pytest.mark.parametrize("precision,tol", [("fp32", POWER_TOL), ("i16", 1e-3)])
+def test_batch_workload_numharm16_autotuned(fd, oracle, precision, tol):
     """Uneven lengths log-uniform in [2^21, 2^23], red noise, some trials with a
-    pulsar; zmax 200, 16 harmonics, fftlen chosen by the autotuner."""
+    pulsar; zmax 200, 16 harmonics, fftlen chosen by the autotuner; FP32 and the
+    reduced-precision mode within its 1e-3 power tolerance (I16 segments and
+    templates, P16 power planes)."""
     from gpu_accelsearch_amd import synth
     lengths = synth.log_uniform_lengths(10, 1 << 21, 1 << 23, seed=4)
     pool = synth.noise_pool(1 << 23, seed=9)
@@ -135,8 +138,9 @@ def test_batch_workload_numharm16_autotuned(fd, oracle):
     for t in with_pulsar:
         xs[t] += synth.pulse_train(lengths[t], 21000.4 + 3000 * t, 11.0, 0.05, 40.0).astype(np.float32)
     powcut = (35.0, 40.0, 50.0, 65.0, 90.0)
-    print_case("batch workload autotune")
-    base = fd.SearchParams(n=lengths[0], zmax=200, numharm=16, fftlen=0, powcut=powcut)
+    print_case("batch workload autotune", precision)
+    base = fd.SearchParams(n=lengths[0], zmax=200, numharm=16, fftlen=0, powcut=powcut,
+                           precision=precision)
     rep = fd.autotune(base, fftlens=(4096, 8192, 16384))
     print("autotune", rep["fftlen"], rep["ms_per_trial"])
     assert rep["fftlen"] in (4096, 8192, 16384)
@@ -155,8 +159,8 @@ def test_batch_workload_numharm16_autotuned(fd, oracle):
             want, stages = ref.step_plane(st, stages=True)
             got = b.run_steps(tr.first_step + st, 1)[0]
             e = float(np.max(np.abs(got - want) / np.maximum(want, 1.0)))
-            print(f"trial {t} step {st} max power error {e:.2e}")
-            assert e <= POWER_TOL
+            print(f"{precision} trial {t} step {st} max power error {e:.2e}")
+            assert e <= tol
             # candidates of that step match the oracle's stage sums away from the margin
             mine = {(int(c["z_index"]), int(c["r_index"]), int(c["stage"]))
                     for c in cands if int(c["trial"]) == t and int(c["step"]) == st}
@@ -164,7 +168,7 @@ def test_batch_workload_numharm16_autotuned(fd, oracle):
             for sg in range(5):
                 z, i = np.nonzero(stages[sg] > powcut[sg])
                 exp |= {(int(a), int(b_), sg) for a, b_ in zip(z, i)}
-                z, i = np.nonzero(np.abs(stages[sg] - powcut[sg]) <= POWER_TOL * np.maximum(stages[sg], 1.0))
+                z, i = np.nonzero(np.abs(stages[sg] - powcut[sg]) <= tol * np.maximum(stages[sg], 1.0))
                 amb |= {(int(a), int(b_), sg) for a, b_ in zip(z, i)}
             assert len(exp) > 0 and mine - amb == exp - amb
             ref.close()

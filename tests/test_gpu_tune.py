@@ -109,16 +109,19 @@ def test_every_harmsum_shape_is_bit_identical(fd):
         L.fdas_set_harmsum_variant(default)
 
 
-# oracle tolerances of the reduced-precision storage modes (as the precision tests)
-PREC_TOL = {"fp16": 2e-3, "bf16": 2e-2, "i16": 1e-3}
+# Oracle bounds of the reduced-precision storage modes: the 1e-3 tolerance for
+# I16, the stated limits of FP16 and BF16 (as the precision tests)
+PREC_TOL = {"fp16": 1.8e-3, "bf16": 1.4e-2, "i16": 1e-3}
 
 
 @pytest.mark.parametrize("precision", ["bf16", "fp16", "i16"])
 @pytest.mark.parametrize("fftlen", [8192, 4096])
 def test_every_corr_variant_reduced_precision(fd, oracle, precision, fftlen):
     """In reduced-precision storage every correlation shape reads the same stored
-    template rows and segments and computes in FP32, so its planes equal the
-    default shape's within FP32 rounding (2e-5 of max(P, 1)).  The split form
+    template rows and segments and computes in FP32, so its powers equal the
+    default shape's to FP32 rounding; stored as P16 (relative step 2^-10) such a
+    difference can move a stored power by one step, so the summed planes agree
+    within 2^-10 of max(P, 1) plus the FP32 rounding (2e-5).  The split form
     stores its own packed bank (sums and differences of template values), so it
     is held to the mode's oracle tolerance instead, as the default shape is."""
     from gpu_accelsearch_amd import synth
@@ -139,7 +142,9 @@ def test_every_corr_variant_reduced_precision(fd, oracle, precision, fftlen):
                 L.fdas_set_corr_variant(l, 0)
             base = [s.run_steps(st, 1)[0].copy() for st in steps]
             for b, w in zip(base, want):
-                assert float(np.max(np.abs(b - w) / np.maximum(w, 1.0))) <= PREC_TOL[precision]
+                e = float(np.max(np.abs(b - w) / np.maximum(w, 1.0)))
+                print(f"TUNE {precision} {fftlen} default shape oracle error {e:.3e}")
+                assert e <= PREC_TOL[precision]
             for v in range(1, nv):
                 split = False
                 for l in lens:
@@ -151,10 +156,11 @@ def test_every_corr_variant_reduced_precision(fd, oracle, precision, fftlen):
                     got = s.run_steps(st, 1)[0]
                     if split:
                         e = float(np.max(np.abs(got - w) / np.maximum(w, 1.0)))
+                        print(f"TUNE {precision} {fftlen} split variant {v} oracle error {e:.3e}")
                         assert e <= PREC_TOL[precision], (v, st, e)
                     else:
                         e = float(np.max(np.abs(got - b) / np.maximum(b, 1.0)))
-                        assert e <= 2e-5, (v, st, e)
+                        assert e <= 2.0 ** -10 + 2e-5, (v, st, e)
     finally:
         for l in ALL_LENGTHS:
             L.fdas_set_corr_variant(l, 0)
