@@ -186,7 +186,7 @@ def serving_form(count, rate, seed=6):
     with fd.Server(max_active=4, chunk_steps=128) as srv:
         for z in (50, 200, 600):
             for h in (4, 8, 16):
-                for prec in ("fp32", "fp16", "bf16"):
+                for prec in ("fp32", "i16"):
                     srv.prewarm(fd.SearchParams(n=1 << 20, zmax=z, numharm=h, fftlen=0, precision=prec))
         ids = [None] * count
         t0 = time.perf_counter()

@@ -15,7 +15,7 @@ MAX_HARM = 16
 MAX_STAGES = 5
 FDAS_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
-          4: "unsupported shape"}
+          4: "unsupported shape", 5: "candidate capacity exceeded"}
 PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "i16": 3}
 LENGTHS = {"shortest": 0, "reference": 1}
 

@@ -407,8 +407,9 @@ class Server:
               "fdas_server_submit")
         return rid.value
 
-    def wait(self, rid: int, capacity: int = 1 << 20):
-        """(result dict, candidates) of request ``rid``; raises if it failed."""
+    def wait(self, rid: int, capacity: int = 1 << 20, raise_on_error: bool = True):
+        """(result dict, candidates) of request ``rid``; raises if it failed, unless
+        raise_on_error is False (then info["status"] holds the error code)."""
         res = N.FdasRequestResult()
         out = np.zeros(capacity, CANDIDATE_DTYPE)
         check(lib().fdas_server_wait(self.handle, rid, C.byref(res),
@@ -416,6 +417,9 @@ class Server:
               "fdas_server_wait")
         info = {k: getattr(res, k) for k, _ in N.FdasRequestResult._fields_ if k != "message"}
         if res.status != 0:
+            if not raise_on_error:
+                info["message"] = res.message.decode()
+                return info, out[:0].copy()
             raise N.FdasError(f"request {rid}: {N.ERRORS.get(res.status, res.status)}: "
                               f"{res.message.decode()}")
         return info, out[:min(res.ncandidates, capacity)].copy()

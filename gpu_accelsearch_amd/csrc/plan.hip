@@ -939,8 +939,15 @@ extern "C" int fdas_get_candidates(fdas_plan* pl, fdas_candidate* out, int capac
     int n = 0;
     FDAS_HIP(hipMemcpyAsync(&n, pl->d_ncand, sizeof(int), hipMemcpyDeviceToHost, pl->stream));
     FDAS_HIP(hipStreamSynchronize(pl->stream));
-    const int kept = std::min(n, pl->cand_cap);
     if (overflowed) *overflowed = n > pl->cand_cap;
+    if (n > pl->cand_cap) {
+        // the stored subset depends on the order the kernels found the cells in
+        *count = n;
+        set_error("%d candidates exceed the capacity of %d; raise max_candidates or the power cuts",
+                  n, pl->cand_cap);
+        return FDAS_ERR_OVERFLOW;
+    }
+    const int kept = n;
     *count = kept;
     if (!out || capacity <= 0) return FDAS_OK;
     std::vector<fdas_candidate> all(kept);

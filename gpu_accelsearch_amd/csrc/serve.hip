@@ -124,6 +124,7 @@ struct Request {
     char message[256] = "";
     std::vector<fdas_candidate> cands;
     int overflowed = 0, nsteps = 0;
+    int nfound = 0;                  // candidates found (more than kept on overflow)
     long long cells = 0;
     size_t bytes = 0;
 };
@@ -198,6 +199,7 @@ void finish(Request* r)
     int count = 0;
     if (r->status == FDAS_OK) {
         int rc = fdas_get_candidates(r->plan, nullptr, 0, &count, &r->overflowed);
+        r->nfound = count;
         if (rc == FDAS_OK) {
             r->cands.resize(count);
             rc = fdas_get_candidates(r->plan, r->cands.data(), count, &count, &r->overflowed);
@@ -462,7 +464,7 @@ extern "C" int fdas_server_wait(fdas_server* S, long long id, fdas_request_resul
     memset(res, 0, sizeof(*res));
     res->id = r->id;
     res->status = r->status;
-    res->ncandidates = (int)r->cands.size();
+    res->ncandidates = r->nfound;
     res->overflowed = r->overflowed;
     res->nsteps = r->nsteps;
     res->cells = r->cells;

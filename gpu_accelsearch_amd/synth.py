@@ -94,9 +94,10 @@ def noise_pool(n: int, seed: int = 5, dt: float = DT) -> np.ndarray:
 
 
 def serving_stream(count: int, rate: float, max_len: int = 1 << 25, min_len: int = 1 << 20,
-                   seed: int = 6, pulsar_frac: float = 0.05):
+                   seed: int = 6, pulsar_frac: float = 0.05, precisions=("fp32", "i16")):
     """Request descriptions of a Poisson stream: arrival time (s), length, zmax,
-    numharm, storage precision and whether a pulsar is injected."""
+    numharm, storage precision (FP32 or the reduced mode within the 1e-3 power
+    tolerance) and whether a pulsar is injected."""
     rng = np.random.default_rng(seed)
     t = np.cumsum(rng.exponential(1.0 / rate, count))
     lens = log_uniform_lengths(count, min_len, max_len, seed + 1)
@@ -104,7 +105,7 @@ def serving_stream(count: int, rate: float, max_len: int = 1 << 25, min_len: int
     for i in range(count):
         out.append({"t": float(t[i]), "n": lens[i], "zmax": int(rng.choice([50, 200, 600])),
                     "numharm": int(rng.choice([4, 8, 16])),
-                    "precision": str(rng.choice(["fp32", "fp16", "bf16"])),
+                    "precision": str(rng.choice(list(precisions))),
                     "pulsar": bool(rng.uniform() < pulsar_frac),
                     "offset": int(rng.integers(0, max_len - lens[i] + 1))})
     return out

@@ -147,7 +147,8 @@ enum {
     FDAS_ERR_INVALID = 1,     /* bad parameter                               */
     FDAS_ERR_HIP = 2,         /* HIP runtime error                           */
     FDAS_ERR_NOMEM = 3,       /* device allocation failed                    */
-    FDAS_ERR_UNSUPPORTED = 4  /* shape outside what the kernels implement    */
+    FDAS_ERR_UNSUPPORTED = 4, /* shape outside what the kernels implement    */
+    FDAS_ERR_OVERFLOW = 5     /* more candidates than the plan's capacity    */
 };
 
 /* Storage of spectra segments and templates (FFT arithmetic, power planes and
@@ -249,7 +250,12 @@ int fdas_run_steps(fdas_plan *plan, int step0, int nsteps, float *d_planes);
 int fdas_search(fdas_plan *plan, const float *d_series);
 /* Copy of the whitened spectrum (nbins complex) to device memory d_out. */
 int fdas_get_spectrum(fdas_plan *plan, fdas_complex *d_out);
-/* Candidates found since the last reset, in (trial, step, z, r, stage) order. */
+/* Candidates found since the last reset, in (trial, step, z, r, stage) order;
+ * *count is their number and the first min(capacity, *count) are written.  If
+ * more cells passed the thresholds than the plan holds (max_candidates), which of
+ * them were stored would depend on the order the GPU found them in, so no list
+ * is given: *count is the number found, *overflowed is 1, and the call returns
+ * FDAS_ERR_OVERFLOW (raise max_candidates or the power cuts). */
 int fdas_get_candidates(fdas_plan *plan, fdas_candidate *out, int capacity, int *count,
                         int *overflowed);
 int fdas_reset_candidates(fdas_plan *plan);
@@ -427,7 +433,8 @@ typedef struct fdas_server_config {
 typedef struct fdas_request_result {
     long long id;
     int status;            /* FDAS_OK or the error that ended the request        */
-    int ncandidates;
+    int ncandidates;       /* candidates found; with status FDAS_ERR_OVERFLOW
+                              (more than the request's capacity) none are returned */
     int overflowed;
     int nsteps;
     long long cells;

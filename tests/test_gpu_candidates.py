@@ -106,3 +106,31 @@ def test_pulse_train_sift_and_optimise(fd):
               opt["opt_power"], opt["harm_power"][:8])
         assert abs(opt["opt_r"] - r1) < 0.02 and abs(opt["opt_z"] - z1) < 0.3
         assert opt["opt_power"] >= 0.97 * top["power"]
+
+
+def test_candidate_overflow_is_a_deterministic_error(fd):
+    """More cells above the thresholds than the plan holds: no partial list (which
+    cells got stored would depend on the order the GPU found them in), the same
+    error and the same count, the true number found, on every run."""
+    import ctypes as C
+    from gpu_accelsearch_amd import synth
+    n = 1 << 20
+    x = synth.plain_series(n, seed=41, r0=20000.3, z0=4.0, power=60.0)
+    powcut = (6.0, 9.0, 14.0, 0.0, 0.0)
+    print_case("candidate overflow")
+    with fd.AccelSearch(fd.SearchParams(n=n, zmax=50, numharm=4, fftlen=2048, powcut=powcut)) as s:
+        s.search(x)
+        total = len(s.candidates())
+    assert total > 1000
+    L = fd.lib()
+    P = fd.SearchParams(n=n, zmax=50, numharm=4, fftlen=2048, powcut=powcut, max_candidates=100)
+    with fd.AccelSearch(P) as s:
+        for _ in range(2):
+            s.search(x)
+            cnt, ov = C.c_int(), C.c_int()
+            buf = (fd._native.FdasCandidate * 100)()
+            rc = L.fdas_get_candidates(s.handle, buf, 100, C.byref(cnt), C.byref(ov))
+            assert rc == 5 and ov.value == 1 and cnt.value == total
+            assert "exceed the capacity" in fd.last_error()
+            with pytest.raises(fd.FdasError, match="candidate capacity exceeded"):
+                s.candidates()
