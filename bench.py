@@ -6,11 +6,15 @@ A "step" of this benchmark is one DM trial of the production workload:
 resident in HBM), R2C FFT, whitening, every overlap-save step (1084 x 201 x 7734
 f-fdot cells), every cell summed over 8 harmonics and tested against the stage
 thresholds, candidates out (fdas_search; summed planes are stored only when a
-caller asks for them with fdas_run_steps).
+caller asks for them with fdas_run_steps).  `value` is that FP32 form's cells/s.
 
-Prints one JSON line: cells/s (value), DM-trials/s, p50/p99 trial latency, the
-roofline of the dominant kernel (HIP events on the plan's stream), and the CPU
-oracle timed on a bounded sample of the same workload on this host.
+Beside it, on the same box: the batched reduced-precision form (I16 segments and
+templates, P16 power planes, hipGraph replay) against the fused FP32 form; the
+16-harmonic uneven-length batch workload; the serving loop on a Poisson request
+stream (at least 500 requests) and a rate sweep; the reference-structured stage
+form; and the CPU baseline (the FP32 CPU form on all host cores over the whole
+trial, the reference's CPU stages timed on their own, and the FP64 oracle).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -18,6 +22,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -25,16 +30,20 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-HBM_PEAK = 8.0e12  # bytes/s, MI355X HBM3E
+HBM_PEAK = 8.0e12        # bytes/s, MI355X HBM3E
+VALU_PEAK = 157.3e12     # FP32 vector FLOP/s (counts packed FP32 instructions)
+VALU_UNPACKED = 78.65e12  # FP32 vector FLOP/s without packed instructions (these kernels)
+POWCUT = (40.0, 45.0, 55.0, 70.0, 0.0)
 
 
-def alg_bytes(info, n: int) -> dict:
+def alg_bytes(info, n: int, pbytes: int = 4) -> dict:
     """Algorithmic HBM bytes per DM trial, and per kernel category.
 
     Trial: series in 4N + spectrum out 8(N/2+1) + per step the segment reads
     4 L_h per harmonic, the sub-harmonic power planes written and re-read
-    2*4*Z_h*R_h, and the summed fundamental plane written once 4*Z_0*U.
-    Templates are resident constants and not counted.
+    2*b*Z_h*R_h, and the fundamental plane written once b*Z_0*U (b = bytes per
+    stored power: 4 in FP32, 2 with P16 planes).  Templates are resident constants
+    and not counted.
     corr_sub:  segment reads + sub-plane writes.
     corr_fund: fundamental segment reads + fundamental power writes.
     harmsum:   sub-plane reads + fundamental plane read (the summed plane is
@@ -43,12 +52,12 @@ def alg_bytes(info, n: int) -> dict:
     S = info.nsteps
     seg = S * sum(4 * info.fftlen[h] for h in range(H))
     sub = sum(info.sub_cells[h] for h in range(1, H))
-    total = 4 * n + 8 * (n // 2 + 1) + seg + 2 * 4 * sub + 4 * info.cells
+    total = 4 * n + 8 * (n // 2 + 1) + seg + 2 * pbytes * sub + pbytes * info.cells
     return {
         "trial": total,
-        "corr_sub": S * sum(4 * info.fftlen[h] for h in range(1, H)) + 4 * sub,
-        "corr_fund": S * 4 * info.fftlen[0] + 4 * info.cells,
-        "harmsum": 4 * sub + 4 * info.cells,
+        "corr_sub": S * sum(4 * info.fftlen[h] for h in range(1, H)) + pbytes * sub,
+        "corr_fund": S * 4 * info.fftlen[0] + pbytes * info.cells,
+        "harmsum": pbytes * sub + pbytes * info.cells,
     }
 
 
@@ -65,8 +74,8 @@ def corr_flops(info) -> dict:
 
 def measured_traffic(kernel: str):
     """HBM bytes per launch of a kernel category from the committed counter
-    summary (two rocprofv3 --pmc runs of this script, FETCH_SIZE and WRITE_SIZE,
-    scaled by the same counters on a copy of known size), or None."""
+    summary of the FP32 production trial (rocprofv3 --pmc FETCH_SIZE, then
+    WRITE_SIZE, scaled by the same counters on a copy of known size), or None."""
     path = os.path.join(ROOT, "profiles", "production_trial_traffic.json")
     try:
         with open(path) as f:
@@ -76,39 +85,72 @@ def measured_traffic(kernel: str):
         return None
 
 
-def cpu_baseline(series, n, zmax, numharm, fftlen, steps):
-    """The CPU oracle (FP64 restatement, one core) on a bounded sample."""
+def host_cores() -> int:
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(series, n, zmax, numharm, fftlen, fp64_steps):
+    """FP32 CPU form (oracle/fdas_cpu32.c) over the whole trial on all host cores,
+    the reference's CPU stages timed on their own, and the FP64 oracle on a
+    contiguous block of steps (all cores) as a secondary figure."""
     from oracle import oracle as O
+    cores = host_cores()
     t0 = time.perf_counter()
+    c = O.Cpu32(series, n, 6.4e-5, zmax, numharm, fftlen)
+    t_create = time.perf_counter() - t0
+    steps = np.arange(c.nsteps)
+    wall, hits, t_norm = c.run_steps(steps, POWCUT, threads=cores)
+    cells = c.nsteps * c.uselen * c.numz
+    per_trial = c.seconds["r2c"] + c.seconds["whiten"] + wall
+    nsteps = c.nsteps
+    c.close()
     ref = O.Search(series, n, 6.4e-5, zmax, numharm, fftlen)
-    t_setup = time.perf_counter() - t0
-    sel = np.linspace(0, ref.nsteps - 1, steps).astype(int)
+    s0 = ref.nsteps // 2
     t0 = time.perf_counter()
-    for st in sel:
-        ref.step_plane(int(st))
-    t_steps = time.perf_counter() - t0
-    cells = len(sel) * ref.uselen * ref.numz
+    ref.candidates(POWCUT, 0.0, s0, fp64_steps)
+    t64 = time.perf_counter() - t0
+    cells64 = fp64_steps * ref.uselen * ref.numz
     ref.close()
-    return {"value": cells / t_steps, "unit": "f-fdot cells/s", "cores": 1, "kind": "port",
-            "sample": f"oracle/liboracle.so (FP64): spectrum+whitening+templates once "
-                      f"({t_setup:.1f} s), then {len(sel)} of {ref.nsteps} steps spread over "
-                      f"the trial ({t_steps:.1f} s); cells/s over the sampled steps"}
+    return {"value": cells / per_trial, "unit": "f-fdot cells/s", "cores": cores, "kind": "port",
+            "impl": "fp32-omp",
+            "sample": f"the whole trial on {cores} host cores: FP32 R2C + whitening + all "
+                      f"{nsteps} steps (radix-4 Stockham FFTs, OpenMP over steps); the template "
+                      "bank, a resident constant as on the GPU, is timed apart",
+            "dm_trials_per_s": 1.0 / per_trial, "ms_per_trial": per_trial * 1e3,
+            "cells_above_cuts": int(hits),
+            "reference_cpu_stages_s": {
+                "deredden": c_seconds(c, "whiten"), "template_bank": c_seconds(c, "templates"),
+                "segment_normalisation_thread_s_per_trial": t_norm,
+                "segment_normalisation_s_per_trial_on_all_cores": t_norm / cores,
+                "r2c_fp32": c_seconds(c, "r2c"), "setup_wall": t_create},
+            "fp64_oracle": {"value": cells64 / t64, "unit": "f-fdot cells/s", "cores": cores,
+                            "sample": f"steps [{s0}, {s0 + fp64_steps}) of the trial, FP64 "
+                                      "FFT correlation and harmonic sum, OpenMP over steps"}}
 
 
-def stage_form(spec, n, zmax, numharm, fftlen, info, steps):
-    """The reference's per-step stage sequence (AccelSearch.cu:217-241) on the GPU:
-    host segment fetch, unfused forward FFTs, complex correlation planes written
-    to memory, batched inverse FFTs, powers and harmonic sums, one step at a time."""
+def c_seconds(c, k):
+    return float(c.seconds[k])
+
+
+def stage_form(spec, zmax, numharm, info, steps):
+    """The reference's per-step stage sequence (AccelSearch.cu:217-241) through
+    the reference-named entry points: host segment fetch, normalise-and-spread,
+    one H2D copy, unfused forward FFTs, complex correlation planes written to
+    memory, batched inverse FFTs, powers and harmonic sums, one step at a time
+    (no per-step allocation; the plane stays on the device)."""
     import gpu_accelsearch_amd as fd
     fd.accel.fdas_set_uselen(info.uselen)
     try:
         bank = fd.createSubharmonicInfo(int(np.log2(numharm)) + 1, zmax)
         pipe = fd.StagePipeline(bank)
         sel = np.linspace(0, info.nsteps - 1, steps).astype(int)
-        pipe.step(spec, info.rlo, -zmax, info.uselen, info.nbins)   # warm-up
+        pipe.step(spec, info.rlo, -zmax, info.uselen, info.nbins, to_host=False)   # warm-up
+        fd.synchronize()
         t0 = time.perf_counter()
         for st in sel:
-            pipe.step(spec, info.rlo + int(st) * (info.uselen // 2), -zmax, info.uselen, info.nbins)
+            pipe.step(spec, info.rlo + int(st) * (info.uselen // 2), -zmax, info.uselen,
+                      info.nbins, to_host=False)
+        fd.synchronize()
         dt = (time.perf_counter() - t0) / len(sel)
         pipe.close()
         bank.free()
@@ -123,7 +165,7 @@ def batch_form(series, n, zmax, numharm, trials, precision, fftlen, reps=3):
     """Many DM trials in one plan, replayed from a captured hipGraph."""
     import gpu_accelsearch_amd as fd
     P = fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen, precision=precision,
-                        powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+                        powcut=POWCUT)
     with fd.BatchSearch(P, [n] * trials, graph=True) as b:
         for t in range(trials):
             b.upload(t, series)
@@ -167,52 +209,84 @@ def batch_workload_form(trials, precision, reps=1):
                 "candidates": ncand, "fftlen_ms_per_trial_at_first_length": rep["ms_per_trial"]}
 
 
-def serving_form(count, rate, seed=6):
-    """A bounded Poisson request stream through the continuous-batching server."""
-    import threading
+def _pct(v, q):
+    return float(np.percentile(v, q)) if len(v) else None
+
+
+def serving_run(srv, pool, reqs, rate):
+    """One Poisson stream through a running server; per-request latencies."""
     import gpu_accelsearch_amd as fd
     from gpu_accelsearch_amd import synth
-    reqs = synth.serving_stream(count, rate, seed=seed)
+    srv.reset_stats()
+    ids = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def feed():
+        for i, r in enumerate(reqs):
+            delay = r["t"] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            x = pool[r["offset"]:r["offset"] + r["n"]]
+            if r["pulsar"]:
+                x = x + synth.pulse_train(r["n"], 0.37 * r["n"] / 2, 0.5 * r["zmax"], 0.05,
+                                          30.0).astype(np.float32)
+            p = fd.SearchParams(n=r["n"], zmax=r["zmax"], numharm=r["numharm"], fftlen=0,
+                                precision=r["precision"], powcut=(40.0, 45.0, 55.0, 70.0, 90.0))
+            ids[i] = srv.submit(p, x)
+
+    th = threading.Thread(target=feed)
+    th.start()
+    th.join()
+    res = [srv.wait(i)[0] for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat = np.array([r["latency_ms"] for r in res])
+    out = {"requests": len(reqs), "offered_rate_per_s": rate, "wall_s": wall,
+           "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
+           "dm_trials_per_s": len(reqs) / wall, "cells_per_s": sum(r["cells"] for r in res) / wall,
+           "max_active_seen": st["max_active_seen"], "memory_peak_bytes": st["memory_peak"],
+           "failed": st["failed"]}
+    classes = {}
+    for lo, hi, name in ((1 << 20, 1 << 22, "n<2^22"), (1 << 22, 1 << 24, "2^22<=n<2^24"),
+                         (1 << 24, (1 << 25) + 1, "n>=2^24")):
+        sel = [k for k, r in enumerate(reqs) if lo <= r["n"] < hi]
+        classes[name] = {"requests": len(sel), "p50_ms": _pct(lat[sel], 50), "p99_ms": _pct(lat[sel], 99)}
+    for z in (50, 200, 600):
+        sel = [k for k, r in enumerate(reqs) if r["zmax"] == z]
+        classes[f"zmax{z}"] = {"requests": len(sel), "p50_ms": _pct(lat[sel], 50),
+                               "p99_ms": _pct(lat[sel], 99)}
+    out["latency_by_class"] = classes
+    return out
+
+
+def serving_form(count, rate, sweep, sweep_count, seed=6):
+    """The serving workload through the continuous-batching server: a Poisson
+    stream of `count` requests at `rate` per second (p50/p99 over them), then
+    shorter streams at each rate of `sweep`; the completed requests per second at
+    the highest offered rates give the server's capacity."""
+    import gpu_accelsearch_amd as fd
+    from gpu_accelsearch_amd import synth
     pool = synth.noise_pool(1 << 25)
-    series = []
-    for r in reqs:
-        x = pool[r["offset"]:r["offset"] + r["n"]].copy()
-        if r["pulsar"]:
-            x += synth.pulse_train(r["n"], 0.37 * r["n"] / 2, 0.5 * r["zmax"], 0.05, 30.0).astype(np.float32)
-        series.append(x)
-    params = [fd.SearchParams(n=r["n"], zmax=r["zmax"], numharm=r["numharm"], fftlen=0,
-                              precision=r["precision"], powcut=(40.0, 45.0, 55.0, 70.0, 90.0))
-              for r in reqs]
     with fd.Server(max_active=4, chunk_steps=128) as srv:
         for z in (50, 200, 600):
             for h in (4, 8, 16):
                 for prec in ("fp32", "i16"):
                     srv.prewarm(fd.SearchParams(n=1 << 20, zmax=z, numharm=h, fftlen=0, precision=prec))
-        ids = [None] * count
-        t0 = time.perf_counter()
-
-        def feed():
-            for i, (r, p, x) in enumerate(zip(reqs, params, series)):
-                delay = r["t"] - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
-                ids[i] = srv.submit(p, x)
-
-        th = threading.Thread(target=feed)
-        th.start()
-        th.join()
-        res = [srv.wait(i)[0] for i in ids]
-        wall = time.perf_counter() - t0
-        st = srv.stats()
-    cells = sum(r["cells"] for r in res)
-    return {"requests": count, "offered_rate_per_s": rate, "wall_s": wall,
-            "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "mean_ms": st["mean_ms"],
-            "dm_trials_per_s": st["trials_per_s"], "cells_per_s": cells / wall,
-            "max_active_seen": st["max_active_seen"], "memory_peak_bytes": st["memory_peak"],
-            "memory_budget_bytes": st["memory_bytes"], "failed": st["failed"],
-            "mix": "n log-uniform [2^20, 2^25], zmax {50,200,600}, numharm {4,8,16}, "
-                   "precision {fp32,fp16,bf16}, 5% with a pulsar; fftlen per request from the "
-                   "autotuned/default table"}
+        main = serving_run(srv, pool, synth.serving_stream(count, rate, seed=seed), rate)
+        runs = []
+        for k, r in enumerate(sweep):
+            s = serving_run(srv, pool, synth.serving_stream(sweep_count, r, seed=seed + 10 + k), r)
+            runs.append({key: s[key] for key in ("offered_rate_per_s", "requests", "wall_s",
+                                                 "dm_trials_per_s", "p50_ms", "p99_ms",
+                                                 "max_active_seen", "failed")})
+        budget = srv.stats()["memory_bytes"]
+    main["memory_budget_bytes"] = budget
+    main["rate_sweep"] = runs
+    main["capacity_dm_trials_per_s"] = max(r["dm_trials_per_s"] for r in runs + [main])
+    main["mix"] = ("n log-uniform [2^20, 2^25], zmax {50,200,600}, numharm {4,8,16}, precision "
+                   "{fp32, i16}, 5% with a pulsar; fftlen per request from the autotuned/default "
+                   "table; max_active 4, 128 steps per launch group")
+    return main
 
 
 def main() -> int:
@@ -220,23 +294,27 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10, help="timed DM trials")
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-steps", type=int, default=96, help="oracle steps sampled")
+    ap.add_argument("--fp64-steps", type=int, default=64,
+                    help="contiguous steps of the FP64 oracle in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "i16"],
-                    help="storage of spectra segments and templates (arithmetic is FP32)")
+                    help="storage of the single-trial form (arithmetic is FP32)")
     ap.add_argument("--batch-trials", type=int, default=8,
-                    help="DM trials of the batched reduced-precision form (0: skip)")
+                    help="DM trials of the batched forms (0: skip)")
     ap.add_argument("--batch-fftlen", type=int, default=0,
                     help="fftlen of the batched reduced-precision form (0: autotuned among "
                          "4096, 8192 and 16384)")
-    ap.add_argument("--stage-steps", type=int, default=12,
+    ap.add_argument("--stage-steps", type=int, default=24,
                     help="steps sampled for the reference-structured stage form (0: skip)")
     ap.add_argument("--batch-workload-trials", type=int, default=256,
                     help="trials of the uneven-length 16-harmonic batch workload (0: skip)")
-    ap.add_argument("--serve-requests", type=int, default=24,
+    ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests of the Poisson serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-sweep", default="100,200",
+                    help="further offered rates (req/s) for the capacity sweep")
+    ap.add_argument("--serve-sweep-requests", type=int, default=150)
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default correlation shapes (no autotuner launches)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -255,8 +333,7 @@ def main() -> int:
     n, zmax, numharm, fftlen = 1 << 23, 200, 8, 8192
     series = synth.production_series(n)
     params = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen,
-                          chunk_steps=args.chunk, precision=args.precision,
-                          powcut=(40.0, 45.0, 55.0, 70.0, 0.0))
+                          chunk_steps=args.chunk, precision=args.precision, powcut=POWCUT)
     # correlation launch shapes measured for this configuration's FFT lengths
     tune = None if args.no_tune else fd.autotune(params, fftlens=(fftlen,))
     s = AccelSearch(params)
@@ -286,15 +363,40 @@ def main() -> int:
     s.enable_timing(False)
 
     info = s.info
-    ab = alg_bytes(info, n)
+    pbytes = 4 if args.precision == "fp32" else 2
+    ab = alg_bytes(info, n, pbytes)
     per_trial = wall / args.steps
     cells_per_s = info.cells / per_trial
     kern = {k: {"launches": v[0] // prof_trials, "ms_per_trial": v[1] / prof_trials}
             for k, v in tm.items()}
-    dom = max(("corr_fund", "corr_sub", "harmsum"), key=lambda k: tm[k][1])
-    dom_ms = tm[dom][1] / max(tm[dom][0], 1)
-    dom_bytes = ab[dom] / max(tm[dom][0] // prof_trials, 1)
-    achieved = dom_bytes / (dom_ms * 1e-3)
+    fl = corr_flops(info)
+
+    def per_launch(cat):
+        launches = max(tm[cat][0] // prof_trials, 1)
+        ms = tm[cat][1] / max(tm[cat][0], 1)
+        return launches, ms
+
+    # the single largest kernel of the trial: the harmonic sum (one launch per
+    # chunk; the correlation categories are several launches of different kernels)
+    hs_launches, hs_ms = per_launch("harmsum")
+    hs_bytes = ab["harmsum"] / hs_launches
+    kernel_rooflines = {"harmsum": {
+        "bound": "hbm", "kernel": "harmsum_kernel", "launches_per_trial": hs_launches,
+        "avg_launch_ms": hs_ms, "alg_bytes_per_launch": hs_bytes,
+        "achieved_GBps": hs_bytes / (hs_ms * 1e-3) / 1e9,
+        "frac": hs_bytes / (hs_ms * 1e-3) / HBM_PEAK}}
+    for cat, kname in (("corr_fund", "corr_sub_kernel (fundamental)"),
+                       ("corr_sub", "corr_sub/corr_odd_kernel (sub-harmonics)")):
+        launches, ms = per_launch(cat)
+        flops = fl[cat] / launches
+        by = ab[cat] / launches
+        rate = flops / (ms * 1e-3)
+        kernel_rooflines[cat] = {
+            "bound": "valu", "kernel": kname, "launches_per_trial": launches,
+            "avg_launch_ms": ms, "flops_per_launch": flops, "achieved_TFLOPs": rate / 1e12,
+            "frac": rate / VALU_PEAK, "frac_of_unpacked_78.6": rate / VALU_UNPACKED,
+            "hbm_GBps": by / (ms * 1e-3) / 1e9, "hbm_frac": by / (ms * 1e-3) / HBM_PEAK}
+    rf = kernel_rooflines["harmsum"]
     out = {
         "metric": "f-fdot cells/s & DM-trials/s (2^23 pts, zmax200, 8 harm); p50/p99 latency",
         "value": cells_per_s,
@@ -316,19 +418,12 @@ def main() -> int:
         "dm_trials_per_s": 1.0 / per_trial,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": measured_traffic(dom),
-                     "alg_bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
-        # FFT-model FLOP rate of the correlations against the FP32 vector peak (157.3
-        # TFLOP/s counts packed FP32, which these kernels are built without, so 78.6 is
-        # the reachable vector rate)
-        "valu": {k: {"tflops": fl / (tm[k][1] / prof_trials * 1e-3) / 1e12,
-                     "frac_of_157.3": fl / (tm[k][1] / prof_trials * 1e-3) / 157.3e12,
-                     "frac_of_78.6_unpacked": fl / (tm[k][1] / prof_trials * 1e-3) / 78.65e12,
-                     "flops_per_trial": fl}
-                 for k, fl in corr_flops(info).items()},
-        "trial_roofline": {"alg_bytes": ab["trial"],
-                           "frac": ab["trial"] / per_trial / HBM_PEAK},
+        "roofline": {"bound": "hbm", "kernel": "harmsum_kernel", "achieved": rf["achieved_GBps"],
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": rf["frac"],
+                     "traffic": measured_traffic("harmsum") if args.precision == "fp32" else None,
+                     "alg_bytes_per_launch": hs_bytes, "avg_launch_ms": hs_ms},
+        "kernel_rooflines": kernel_rooflines,
+        "trial_roofline": {"alg_bytes": ab["trial"], "frac": ab["trial"] / per_trial / HBM_PEAK},
         "kernels": kern,
         "candidates": int(len(cands)),
         "autotune": None if tune is None else
@@ -338,32 +433,38 @@ def main() -> int:
                   for k, v in tune["corr"].items()},
          "harmsum": {k: tune["harmsum"][k] for k in ("threads", "rows", "cols")}},
         "fft_lengths": [h[2] for h in s.harmonics],
+        "host_cores": host_cores(),
     }
     spec = s.spectrum() if args.stage_steps > 0 else None
     s.close()
     del d_series
-    if args.stage_steps > 0:
-        out["stage_form_fp32"] = stage_form(spec, n, zmax, numharm, fftlen, info, args.stage_steps)
     if args.batch_trials > 0:
         if args.batch_fftlen > 0:
             bt = {"fftlen": args.batch_fftlen, "ms_per_trial": None}
         else:
-            rp = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=0, precision="bf16")
+            rp = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=0, precision="i16")
             bt = fd.autotune(rp, fftlens=(4096, 8192, 16384))
-        out["batch_bf16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "bf16", bt["fftlen"])
-        out["batch_bf16"]["fftlen_ms_per_trial"] = bt["ms_per_trial"]
-        out["batch_i16"] = batch_form(series, n, zmax, numharm, args.batch_trials, "i16", bt["fftlen"])
-        out["batch_fp32"] = batch_form(series, n, zmax, numharm, args.batch_trials, "fp32", fftlen)
-        ref = out.get("stage_form_fp32", {}).get("dm_trials_per_s")
-        out["batch_bf16"]["vs_fused_fp32_single"] = out["batch_bf16"]["dm_trials_per_s"] * per_trial
-        if ref:
-            out["batch_bf16"]["vs_stage_form_fp32"] = out["batch_bf16"]["dm_trials_per_s"] / ref
+        bi = batch_form(series, n, zmax, numharm, args.batch_trials, "i16", bt["fftlen"])
+        bi["fftlen_ms_per_trial"] = bt["ms_per_trial"]
+        bi["storage"] = "I16 segments and templates, P16 power planes (within the 1e-3 tolerance)"
+        bf = batch_form(series, n, zmax, numharm, args.batch_trials, "fp32", fftlen)
+        out["batch_i16"] = bi
+        out["batch_fp32"] = bf
+        out["ratio_batched_reduced_vs_fused_fp32_single"] = bi["dm_trials_per_s"] * per_trial
+        out["ratio_batched_reduced_vs_batched_fp32"] = bi["dm_trials_per_s"] / bf["dm_trials_per_s"]
+    if args.stage_steps > 0:
+        out["stage_form_fp32"] = stage_form(spec, zmax, numharm, info, args.stage_steps)
+        if "batch_i16" in out:
+            out["ratio_batched_reduced_vs_stage_form_fp32"] = (
+                out["batch_i16"]["dm_trials_per_s"] / out["stage_form_fp32"]["dm_trials_per_s"])
     if args.batch_workload_trials > 0:
-        out["batch_workload_bf16"] = batch_workload_form(args.batch_workload_trials, "bf16")
+        out["batch_workload_i16"] = batch_workload_form(args.batch_workload_trials, "i16")
     if args.serve_requests > 0:
-        out["serving"] = serving_form(args.serve_requests, args.serve_rate)
+        sweep = [float(v) for v in args.serve_sweep.split(",") if v]
+        out["serving"] = serving_form(args.serve_requests, args.serve_rate, sweep,
+                                      args.serve_sweep_requests)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(series, n, zmax, numharm, fftlen, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(series, n, zmax, numharm, fftlen, args.fp64_steps)
         out["cpu_baseline"]["vs_gpu"] = cells_per_s / out["cpu_baseline"]["value"]
     if args.copy_probe_mib > 0:
         from gpu_accelsearch_amd import lib

@@ -712,11 +712,13 @@ class StagePipeline:
         self.d_fft = DeviceArray((self.total_fft,), np.complex64)
         self.d_conv = DeviceArray((self.total_conv,), np.complex64)
         self.d_ifft = DeviceArray((self.total_conv,), np.complex64)
+        self.d_pow = None
         lib().initialize_cuda_plans(H, bank.ptr)
 
     def step(self, spectrum: np.ndarray, startr: float, zlo: float, uselen: int,
-             nbins: int, stage: str = "summed") -> np.ndarray:
-        """One step from a host whitened spectrum; returns the power plane."""
+             nbins: int, stage: str = "summed", to_host: bool = True):
+        """One step from a host whitened spectrum; returns the power plane (on
+        the host, or the pipeline's device plane when to_host is False)."""
         H = self.H
         s0, s1 = step_bounds(startr, uselen)
         info, bo = generate_harmonics_ffdot(self.bank, s0, s1, zlo)
@@ -733,15 +735,19 @@ class StagePipeline:
                                   self.d_conv.ptr, self.convIdx)
         lib().perform_inversefft(H, self.d_conv.ptr, self.convIdx, self.d_ifft.ptr)
         numz, numrs = info[0].numzs, info[0].numrs
-        d_pow = DeviceArray((numz, numrs), np.float32)
+        if self.d_pow is None or self.d_pow.shape != (numz, numrs):
+            self.d_pow = DeviceArray((numz, numrs), np.float32)
+        d_pow = self.d_pow
         lib().generate_fundamental_powers(H, self.d_ifft.ptr, self.convIdx, info, self.bank.ptr,
                                           d_pow.ptr)
         if stage == "summed":
             lib().perform_harmonic_summing(H, self.d_ifft.ptr, self.convIdx, info, self.bank.ptr,
                                            d_pow.ptr, int(startr), None, None, 0)
-        synchronize()
         if N.last_error():
             raise N.FdasError(N.last_error())
+        if not to_host:
+            return d_pow
+        synchronize()
         return d_pow.to_host()
 
     def close(self) -> None:

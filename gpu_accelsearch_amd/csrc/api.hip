@@ -365,6 +365,32 @@ extern "C" void generate_harmonics_ffdot(unsigned short totalHarmonics, unsigned
     }
 }
 
+// Device staging buffers of the stage entry points, per device and purpose: kept
+// for the life of the process and grown only when a call needs more, so calls
+// in a loop allocate nothing.
+enum { kStageAmps = 0, kStageSpread = 1, kStageFftWork = 2, kStageKinds = 3 };
+
+static int stage_buffer(int kind, size_t bytes, void** out)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, std::pair<void*, size_t>> bufs;
+    int dev = 0;
+    FDAS_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto& b = bufs[std::make_pair(dev, kind)];
+    if (b.second < bytes) {
+        if (b.first) {
+            FDAS_HIP(hipDeviceSynchronize());   // earlier calls may still read it
+            (void)hipFree(b.first);
+            b = {nullptr, 0};
+        }
+        FDAS_HIP(hipMalloc(&b.first, bytes));
+        b.second = bytes;
+    }
+    *out = b.first;
+    return FDAS_OK;
+}
+
 // source/bpd_search.cu:130-184: fetch (host copy), then median normalisation and
 // the 2x spread on the GPU; the spread rows land in output_data (host).
 extern "C" void generate_harmonic_ffdot_data(unsigned short totalHarmonics, long totalDataLength,
@@ -386,16 +412,15 @@ extern "C" void generate_harmonic_ffdot_data(unsigned short totalHarmonics, long
             return;
         }
         fdas_complex* amps = get_fourier_amplitudes(lobin, nice, fft, dataNumbins, dataLoBin);
-        cplx *d_amps = nullptr, *d_out = nullptr;
-        bool ok = hipMalloc(&d_amps, sizeof(cplx) * nice) == hipSuccess &&
-                  hipMalloc(&d_out, sizeof(cplx) * L) == hipSuccess &&
+        void *d_amps = nullptr, *d_out = nullptr;
+        bool ok = stage_buffer(kStageAmps, sizeof(cplx) * nice, &d_amps) == FDAS_OK &&
+                  stage_buffer(kStageSpread, sizeof(cplx) * L, &d_out) == FDAS_OK &&
                   hipMemcpy(d_amps, amps, sizeof(cplx) * nice, hipMemcpyHostToDevice) == hipSuccess &&
-                  launch_normalize_spread(d_amps, numdata, nhalf, d_out, nullptr, 0) == FDAS_OK &&
+                  launch_normalize_spread(static_cast<const cplx*>(d_amps), numdata, nhalf,
+                                          static_cast<cplx*>(d_out), nullptr, 0) == FDAS_OK &&
                   hipMemcpy(output_data + dataStartIdx[h], d_out, sizeof(cplx) * L,
                             hipMemcpyDeviceToHost) == hipSuccess;
         free(amps);
-        (void)hipFree(d_amps);
-        (void)hipFree(d_out);
         if (!ok) {
             if (!last_error()[0]) set_error("generate_harmonic_ffdot_data: device failure");
             return;
@@ -422,13 +447,11 @@ extern "C" void free_cuda_plans(unsigned short totalHarmonics)
 static int row_fft_any(const cplx* in, cplx* out, int L, int rows, int dir)
 {
     if (L <= 16384) return fft_rows(in, out, L, rows, L, L, dir, 1.0f, 0, 0);
-    cplx* work = nullptr;
-    FDAS_HIP(hipMalloc(&work, 2 * sizeof(cplx) * (size_t)L));
+    void* work = nullptr;
+    FDAS_TRY(stage_buffer(kStageFftWork, 2 * sizeof(cplx) * (size_t)L, &work));
     int rc = FDAS_OK;
     for (int r = 0; r < rows && rc == FDAS_OK; ++r)
-        rc = fft_c2c(in + (size_t)r * L, out + (size_t)r * L, L, dir, work, 0);
-    (void)hipDeviceSynchronize();
-    (void)hipFree(work);
+        rc = fft_c2c(in + (size_t)r * L, out + (size_t)r * L, L, dir, static_cast<cplx*>(work), 0);
     return rc;
 }
 
@@ -493,10 +516,6 @@ extern "C" void perform_harmonic_summing(unsigned short totalHarmonics, fdas_com
     (void)host_rinds;
     (void)device_rinds;
     (void)totalRindsBins;
-    if (hipDeviceSynchronize() != hipSuccess) {
-        set_error("perform_harmonic_summing: device error before summing");
-        return;
-    }
     for (int h = 1; h < totalHarmonics; ++h) {
         const double f = (double)hi[h].currentHarmonic / (double)hi[h].maxHarmonic;
         const int off = shi[h].kernelHalfWidth[0] * NUMBETWEEN;

@@ -637,6 +637,7 @@ or_ctx *or_create(const float *series, const or_params *p)
         c->L[h] = or_fft_length(c->uselen, mx[h], cur[h], p->zmax);
         int L = c->L[h];
         c->K[h] = (double *)malloc(sizeof(double) * 2 * (size_t)L * c->Z[h]);
+#pragma omp parallel for schedule(dynamic)
         for (int k = 0; k < c->Z[h]; k++) {
             double *row = c->K[h] + 2 * (size_t)L * k;
             or_template_row(-c->zmax_h[h] + OR_DZ * k, L, row);
@@ -717,6 +718,77 @@ void or_step_plane(const or_ctx *c, int step, double *out, double *stage_out)
     }
     free(rowbuf);
     free(amps);
+}
+
+/* Candidates of steps [step0, step0 + nsteps): every cell whose partial sum at a
+ * stage exceeds powcut[stage] (a cut <= 0 disables its stage), as in the search
+ * hook of source/bpd_search.cu:350-356; cells whose stage sum lies within
+ * margin * max(P, 1) of a cut are returned flagged ambiguous instead, whichever
+ * side they fall on.  Steps run in parallel (OpenMP); the list comes in step
+ * order, then z, r, stage.  Returns the number of entries (at most cap are
+ * written), or -1 if out of memory. */
+long or_search_candidates(const or_ctx *c, int step0, int nsteps, const float *powcut,
+                          double margin, or_cand *out, long cap)
+{
+    int U = c->uselen, Z0 = c->Z[0], stages = or_fast_log2(c->numharm) + 1;
+    or_cand **per = (or_cand **)calloc((size_t)(nsteps > 0 ? nsteps : 1), sizeof(or_cand *));
+    long *cnt = (long *)calloc((size_t)(nsteps > 0 ? nsteps : 1), sizeof(long));
+    int fail = 0;
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < nsteps; k++) {
+        size_t cells = (size_t)Z0 * U;
+        double *plane = (double *)malloc(sizeof(double) * cells);
+        double *st = (double *)malloc(sizeof(double) * cells * (size_t)stages);
+        long n = 0, capk = 64;
+        or_cand *lst = (or_cand *)malloc(sizeof(or_cand) * (size_t)capk);
+        if (!plane || !st || !lst) {
+#pragma omp atomic write
+            fail = 1;
+        } else {
+            or_step_plane(c, step0 + k, plane, st);
+            for (int j = 0; j < Z0; j++)
+                for (int i = 0; i < U; i++)
+                    for (int s = 0; s < stages; s++) {
+                        double thr = powcut[s];
+                        if (!(thr > 0.0)) continue;
+                        double P = st[(size_t)s * cells + (size_t)j * U + i];
+                        int amb = fabs(P - thr) <= margin * (P > 1.0 ? P : 1.0);
+                        if (!amb && !(P > thr)) continue;
+                        if (n == capk) {
+                            capk *= 2;
+                            or_cand *g = (or_cand *)realloc(lst, sizeof(or_cand) * (size_t)capk);
+                            if (!g) {
+#pragma omp atomic write
+                                fail = 1;
+                                capk /= 2;
+                                n--;   /* the entry below overwrites the last one */
+                            } else {
+                                lst = g;
+                            }
+                        }
+                        or_cand *e = &lst[n++];
+                        e->step = step0 + k;
+                        e->z_index = j;
+                        e->r_index = i;
+                        e->stage = s;
+                        e->power = P;
+                        e->ambiguous = amb;
+                    }
+        }
+        free(plane);
+        free(st);
+        per[k] = lst;
+        cnt[k] = n;
+    }
+    long total = 0;
+    for (int k = 0; k < nsteps; k++) {
+        for (long q = 0; q < cnt[k]; q++, total++)
+            if (total < cap && out) out[total] = per[k][q];
+        free(per[k]);
+    }
+    free(per);
+    free(cnt);
+    return fail ? -1 : total;
 }
 
 /* ------------------------------------------------------------------ optimiser

@@ -100,6 +100,18 @@ const float *or_ctx_spectrum(const or_ctx *c);  /* whitened, nbins complex */
  * per-stage partial-sum planes stage_out[s][z*U+i] (NULL to skip). */
 void    or_step_plane(const or_ctx *c, int step, double *out, double *stage_out);
 
+/* Search cells above the stage cuts over steps [step0, step0 + nsteps), OpenMP over
+ * steps; entries within margin * max(P, 1) of a cut carry ambiguous = 1.  Order:
+ * step, then z, r, stage.  Returns the entry count (first cap written), -1 on
+ * allocation failure. */
+typedef struct {
+    int step, z_index, r_index, stage;
+    double power;       /* stage sum (FP64)                                 */
+    int ambiguous;      /* |power - cut| <= margin * max(power, 1)          */
+} or_cand;
+long    or_search_candidates(const or_ctx *c, int step0, int nsteps, const float *powcut,
+                             double margin, or_cand *out, long cap);
+
 /* ---- candidate optimiser response (build-defined; the reference's optimiser is
  * a stub, source/bpd_candidates.cu:22-26).  R(q, z) = int_0^1 exp(2 pi i (q u +
  * z u^2 / 2)) du by composite Gauss-Legendre quadrature (not the Fresnel form the
@@ -107,6 +119,18 @@ void    or_step_plane(const or_ctx *c, int step, double *out, double *stage_out)
  * |sum_j a_j conj(R(r - z/2 - j, z))|^2 over integers j in [r - hw(z), r + hw(z)). */
 void    or_rz_response(double q, double z, double *re, double *im);
 double  or_rz_power(const float *spec, long nbins, double r, double z);
+
+/* ---- FP32 CPU form (fdas_cpu32.c): the CPU baseline.  The reference's pipeline
+ * in single precision with OpenMP over steps; power-of-two n/2 only. ---- */
+typedef struct c32_ctx c32_ctx;
+void      c32_fft(float *x, int n, int dir);   /* radix-4 Stockham, unnormalised */
+c32_ctx  *c32_create(const float *series, const or_params *p);
+void      c32_destroy(c32_ctx *c);
+int       c32_info(const c32_ctx *c, int *uselen, int *nsteps, int *numz0, double *t_r2c,
+                   double *t_whiten, double *t_templates);
+void      c32_step_plane(const c32_ctx *c, int step, float *out);
+double    c32_run_steps(const c32_ctx *c, const int *steps, int count, int nthreads,
+                        const float *powcut, long *hits, double *t_norm);
 
 #ifdef __cplusplus
 }

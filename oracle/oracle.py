@@ -108,6 +108,22 @@ def lib():
         L.or_ctx_spectrum.argtypes = [C.c_void_p]
         L.or_step_plane.restype = None
         L.or_step_plane.argtypes = [C.c_void_p, C.c_int, _dp, C.c_void_p]
+        L.c32_fft.restype = None
+        L.c32_fft.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.c32_create.restype = C.c_void_p
+        L.c32_create.argtypes = [_fp, C.c_void_p]
+        L.c32_destroy.restype = None
+        L.c32_destroy.argtypes = [C.c_void_p]
+        L.c32_info.restype = C.c_int
+        L.c32_info.argtypes = [C.c_void_p] + [C.POINTER(C.c_int)] * 3 + [C.POINTER(C.c_double)] * 3
+        L.c32_step_plane.restype = None
+        L.c32_step_plane.argtypes = [C.c_void_p, C.c_int, _fp]
+        L.c32_run_steps.restype = C.c_double
+        L.c32_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                    C.POINTER(C.c_long), C.POINTER(C.c_double)]
+        L.or_search_candidates.restype = C.c_long
+        L.or_search_candidates.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_float),
+                                           C.c_double, C.c_void_p, C.c_long]
         L.or_rz_response.restype = None
         L.or_rz_response.argtypes = [C.c_double, C.c_double, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
@@ -196,6 +212,11 @@ def segment_amplitudes(spec_c64: np.ndarray, lobin: int, numdata: int, L: int):
     return out.view(np.complex64), med
 
 
+OR_CAND_DTYPE = np.dtype([("step", np.int32), ("z_index", np.int32), ("r_index", np.int32),
+                          ("stage", np.int32), ("power", np.float64), ("ambiguous", np.int32),
+                          ("_pad", np.int32)])
+
+
 class Search:
     """Whole-path oracle: spectrum, whitening, templates, per-step planes."""
 
@@ -229,6 +250,22 @@ class Search:
             return out, st.reshape(nst, self.numz, self.uselen)
         return out
 
+    def candidates(self, powcut, margin: float, step0: int = 0, nsteps: int | None = None):
+        """Cells above the stage cuts over steps [step0, step0 + nsteps) (OpenMP over
+        steps): structured array step, z_index, r_index, stage, power, ambiguous
+        (the stage sum lies within margin * max(P, 1) of its cut)."""
+        ns = self.nsteps - step0 if nsteps is None else nsteps
+        cut = (C.c_float * 5)(*[float(v) for v in list(powcut) + [0.0] * (5 - len(powcut))])
+        n = lib().or_search_candidates(self.h, int(step0), int(ns), cut, float(margin), None, 0)
+        if n < 0:
+            raise MemoryError("or_search_candidates")
+        out = np.zeros(n, OR_CAND_DTYPE)
+        if n:
+            m = lib().or_search_candidates(self.h, int(step0), int(ns), cut, float(margin),
+                                           out.ctypes.data_as(C.c_void_p), n)
+            assert m == n
+        return out
+
     def close(self):
         if self.h:
             lib().or_destroy(self.h)
@@ -239,6 +276,57 @@ class Search:
             self.close()
         except Exception:
             pass
+
+
+class Cpu32:
+    """FP32 CPU form of the search (fdas_cpu32.c), the CPU baseline: R2C,
+    whitening and template bank at construction (each timed), steps in parallel."""
+
+    def __init__(self, series: np.ndarray, n: int, dt: float, zmax: int, numharm: int,
+                 fftlen: int, uselen: int = 0):
+        self.series = np.ascontiguousarray(series, dtype=np.float32)
+        self._p = OrParams(n, dt, zmax, numharm, fftlen, uselen)
+        self.h = lib().c32_create(self.series, C.byref(self._p))
+        if not self.h:
+            raise ValueError("c32_create: n/2 must be a power of two")
+        u, ns, z0 = C.c_int(), C.c_int(), C.c_int()
+        tr, tw, tt = C.c_double(), C.c_double(), C.c_double()
+        lib().c32_info(self.h, C.byref(u), C.byref(ns), C.byref(z0), C.byref(tr), C.byref(tw),
+                       C.byref(tt))
+        self.uselen, self.nsteps, self.numz = u.value, ns.value, z0.value
+        self.seconds = {"r2c": tr.value, "whiten": tw.value, "templates": tt.value}
+
+    def step_plane(self, step: int) -> np.ndarray:
+        out = np.zeros(self.numz * self.uselen, np.float32)
+        lib().c32_step_plane(self.h, int(step), out)
+        return out.reshape(self.numz, self.uselen)
+
+    def run_steps(self, steps, powcut=(0.0,) * 5, threads: int = 0):
+        """(wall seconds, cells above a stage cut, normalisation thread-seconds)."""
+        st = np.ascontiguousarray(steps, dtype=np.int32)
+        cut = np.array(list(powcut) + [0.0] * (5 - len(powcut)), np.float32)
+        hits, tn = C.c_long(), C.c_double()
+        wall = lib().c32_run_steps(self.h, st.ctypes.data, len(st), int(threads), cut.ctypes.data,
+                                   C.byref(hits), C.byref(tn))
+        return wall, hits.value, tn.value
+
+    def close(self):
+        if self.h:
+            lib().c32_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fft32(x: np.ndarray, sign: int = -1) -> np.ndarray:
+    """The FP32 CPU form's complex FFT (power-of-two length)."""
+    a = np.ascontiguousarray(x, dtype=np.complex64).copy()
+    lib().c32_fft(a.ctypes.data, len(a), int(sign))
+    return a
 
 
 def rz_response(q: float, z: float) -> complex:

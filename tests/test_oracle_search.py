@@ -78,3 +78,33 @@ def test_harmonic_sum_collects_planted_harmonics(oracle):
     assert abs(S.rlo + k * (S.uselen // 2) + 0.5 * i - 4 * r1) <= 0.5
     assert plane[j, i] > 2.0 * st[0][j, i]
     S.close()
+
+
+def test_search_candidates_match_stage_planes(oracle):
+    """or_search_candidates (OpenMP over steps) against the stage partial sums of
+    or_step_plane, step by step: the cells above each cut, and the cells within
+    the margin of a cut flagged ambiguous."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 18
+    x = synth.plain_series(n, seed=17, r0=9000.3, z0=6.0, power=70.0)
+    ref = oracle.Search(x, n, 6.4e-5, 30, 4, 1024)
+    cut, margin = (14.0, 18.0, 26.0, 0.0, 0.0), 0.05
+    cands = ref.candidates(cut, margin)
+    assert len(cands) > 0 and cands["ambiguous"].sum() > 0
+    assert np.all(np.diff(cands["step"]) >= 0)
+    for st in range(ref.nsteps):
+        _, S = ref.step_plane(st, stages=True)
+        above, amb = set(), set()
+        for s in range(3):
+            P = S[s]
+            near = np.abs(P - cut[s]) <= margin * np.maximum(P, 1.0)
+            z, i = np.nonzero(near)
+            amb |= {(int(a), int(b), s) for a, b in zip(z, i)}
+            z, i = np.nonzero((P > cut[s]) & ~near)
+            above |= {(int(a), int(b), s) for a, b in zip(z, i)}
+        mine = cands[cands["step"] == st]
+        assert {(int(c["z_index"]), int(c["r_index"]), int(c["stage"])) for c in mine
+                if not c["ambiguous"]} == above
+        assert {(int(c["z_index"]), int(c["r_index"]), int(c["stage"])) for c in mine
+                if c["ambiguous"]} == amb
+    ref.close()
