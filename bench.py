@@ -312,7 +312,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=500,
                     help="requests of the Poisson serving stream (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
-    ap.add_argument("--serve-sweep", default="100,200",
+    ap.add_argument("--serve-sweep", default="20,100,200",
                     help="further offered rates (req/s) for the capacity sweep")
     ap.add_argument("--serve-sweep-requests", type=int, default=150)
     ap.add_argument("--no-tune", action="store_true",

@@ -55,6 +55,21 @@ inline int odd_factor(int L)
     return 1;
 }
 
+// exp(2 pi i idx / M), 0 <= idx < M, as hi[idx >> shift] * lo[idx & (2^shift - 1)]:
+// both tables hold values computed in double on the host and rounded to float once
+// (about sqrt(M) entries each; cached per device and M for the life of the process),
+// so kernels need no double-precision trigonometry.
+struct PhaseTable {
+    const cplx* lo = nullptr;
+    const cplx* hi = nullptr;
+    int shift = 0;
+};
+int phase_table(long long M, PhaseTable* out);
+__device__ __forceinline__ cplx phase_at(const PhaseTable& t, long long idx)
+{
+    return cmul(t.hi[idx >> t.shift], t.lo[idx & ((1LL << t.shift) - 1)]);
+}
+
 // Row transforms: out[row*out_stride + k] = scale * DFT_dir(in[row*in_stride + .])[k]
 // (times exp(dir 2 pi i (row*k mod tw_m)/tw_m) when tw_m > 0).  In place allowed.
 int fft_rows(const cplx* in, cplx* out, int n, long nrows, long in_stride, long out_stride,
@@ -206,14 +221,18 @@ int harmsum_variant_count();
 int harmsum_variant();
 int set_harmsum_variant(int v);
 void harmsum_variant_shape(int v, int* threads, int* rows, int* cols);
+// variant < 0: the shape currently selected process-wide.
 int launch_harmsum(const void* d_fund, float* d_out, const void* d_sub, int prec, const HarmSumArgs& hs,
                    int step0, int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
-                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s);
+                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s, int variant = -1);
 
 // Plan creation on a given stream (0: the plan makes its own) with buffers from
 // al (nullptr: hipMalloc).  Implemented in plan.hip.
 int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, bool own_series,
                    hipStream_t stream, DevAlloc* al, fdas_plan** out);
+
+// A plan's harmonic-sum tile shape (it starts with the process-wide one).
+int plan_set_harmsum_variant(fdas_plan* pl, int v);
 
 // Fundamental FFT length for params (p.fftlen if set; tuned or default else).
 int resolve_fftlen(const fdas_params& p);

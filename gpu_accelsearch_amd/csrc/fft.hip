@@ -6,6 +6,7 @@
 // into their store, transpose, m1 row transforms of length m2, transpose.
 // A real series of n floats is read as n/2 complex values (even/odd pairs),
 // transformed, and split into the n/2+1 bins of the real transform.
+#include <cmath>
 #include <map>
 #include <mutex>
 
@@ -149,6 +150,48 @@ const cplx* fft_odd_twiddles(int r, int m, int* err)
     return d;
 }
 
+int phase_table(long long M, PhaseTable* out)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, long long>, PhaseTable> cache;
+    if (M < 1) {
+        set_error("phase table of modulus %lld", M);
+        return FDAS_ERR_INVALID;
+    }
+    int dev = 0;
+    FDAS_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, M);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        *out = it->second;
+        return FDAS_OK;
+    }
+    int shift = 0;
+    while ((1LL << (2 * shift)) < M) ++shift;   // 2^shift >= sqrt(M)
+    const long long nlo = 1LL << shift, nhi = (M + nlo - 1) / nlo;
+    std::vector<cplx> h((size_t)(nlo + nhi));
+    const double two_pi = 6.283185307179586476925286766559;
+    for (long long i = 0; i < nlo; ++i) {
+        const double a = two_pi * (double)i / (double)M;
+        h[(size_t)i] = mk((float)std::cos(a), (float)std::sin(a));
+    }
+    for (long long i = 0; i < nhi; ++i) {
+        const double a = two_pi * (double)((i * nlo) % M) / (double)M;
+        h[(size_t)(nlo + i)] = mk((float)std::cos(a), (float)std::sin(a));
+    }
+    cplx* d = nullptr;
+    FDAS_HIP(hipMalloc(&d, h.size() * sizeof(cplx)));
+    FDAS_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice));
+    PhaseTable t;
+    t.lo = d;
+    t.hi = d + nlo;
+    t.shift = shift;
+    cache[key] = t;
+    *out = t;
+    return FDAS_OK;
+}
+
 // ------------------------------------------------------------------ kernels
 // Forward transform of rows of length R*M into the decimated order (template
 // banks of the length-R*M correlations).  Every thread's loads precede the first
@@ -225,7 +268,7 @@ int fft_odd_rows(const cplx* in, cplx* out, int r, int m, long nrows, hipStream_
 template <int N, int DIR>
 __global__ __launch_bounds__(FftCfg<N>::T) void fft_rows_kernel(
     const cplx* __restrict__ in, cplx* __restrict__ out, long in_stride, long out_stride,
-    const cplx* __restrict__ tw, float scale, long nrows, long tw_m)
+    const cplx* __restrict__ tw, float scale, long nrows, long tw_m, PhaseTable ph)
 {
     constexpr int T = FftCfg<N>::T;
     using F = BlockFFT<N, T, DIR>;
@@ -244,10 +287,9 @@ __global__ __launch_bounds__(FftCfg<N>::T) void fft_rows_kernel(
         const int k = F::out_index(t, i);
         cplx y = scale == 1.0f ? v[i] : mk(v[i].x * scale, v[i].y * scale);
         if (tw_m > 0) {
-            const long long m = ((long long)row * k) % tw_m;
-            double sn, cs;
-            sincospi(2.0 * (double)m / (double)tw_m, &sn, &cs);
-            y = cmul(y, mk((float)cs, (float)(DIR * sn)));
+            cplx w = phase_at(ph, ((long long)row * k) % tw_m);
+            if (DIR < 0) w.y = -w.y;
+            y = cmul(y, w);
         }
         dst[k] = y;
     }
@@ -274,7 +316,7 @@ __global__ __launch_bounds__(256) void transpose_kernel(const cplx* __restrict__
 // X[k] = (Z[k] + conj Z[m-k])/2 - i/2 exp(-2 pi i k / 2m) (Z[k] - conj Z[m-k]),
 // k = 0..m, for Z the m-point transform of z[j] = x[2j] + i x[2j+1].
 __global__ __launch_bounds__(256) void r2c_split_kernel(const cplx* __restrict__ Z,
-                                                        cplx* __restrict__ X, long m)
+                                                        cplx* __restrict__ X, long m, PhaseTable ph)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k > m) return;
@@ -284,9 +326,8 @@ __global__ __launch_bounds__(256) void r2c_split_kernel(const cplx* __restrict__
     const cplx fe = mk(0.5f * (a.x + b.x), 0.5f * (a.y + b.y));
     const cplx d = mk(a.x - b.x, a.y - b.y);
     const cplx fo = mk(0.5f * d.y, -0.5f * d.x);
-    double sn, cs;
-    sincospi(-(double)k / (double)m, &sn, &cs);
-    const cplx w = mk((float)cs, (float)sn);
+    cplx w = phase_at(ph, k);   // exp(-pi i k / m): the table has modulus 2m
+    w.y = -w.y;
     X[k] = fe + cmul(w, fo);
 }
 
@@ -299,13 +340,15 @@ static int launch_rows(const cplx* in, cplx* out, long nrows, long in_stride, lo
     const cplx* tw = fft_twiddles(N, &err);
     if (!tw) return err;
     if (nrows <= 0) return FDAS_OK;
+    PhaseTable ph;
+    if (tw_m > 0) FDAS_TRY(phase_table(tw_m, &ph));
     dim3 grid((unsigned)nrows), block(FftCfg<N>::T);
     if (dir < 0)
         hipLaunchKernelGGL((fft_rows_kernel<N, -1>), grid, block, 0, s, in, out, in_stride,
-                           out_stride, tw, scale, nrows, tw_m);
+                           out_stride, tw, scale, nrows, tw_m, ph);
     else
         hipLaunchKernelGGL((fft_rows_kernel<N, 1>), grid, block, 0, s, in, out, in_stride,
-                           out_stride, tw, scale, nrows, tw_m);
+                           out_stride, tw, scale, nrows, tw_m, ph);
     return check_launch("fft_rows_kernel");
 }
 
@@ -363,32 +406,32 @@ int fft_c2c(const cplx* in, cplx* out, long m, int dir, cplx* work, hipStream_t 
 // ------------------------------------------------------------------ Bluestein
 // m-point forward DFT of any length through power-of-two transforms of length
 // p >= 2m-1:  Z[k] = conj(c_k) * sum_j (z_j conj(c_j)) c_{k-j},  c_j = exp(i pi j^2/m).
-// The chirp phase uses j^2 mod 2m in integers, so it is exact for any m.
-__device__ __forceinline__ cplx chirp(long long j, long long m)
+// The chirp phase uses j^2 mod 2m in integers, so it is exact for any m; the
+// phase table has modulus 2m.
+__device__ __forceinline__ cplx chirp(long long j, long long m, const PhaseTable& ph)
 {
-    const long long q = (j * j) % (2 * m);
-    double sn, cs;
-    sincospi((double)q / (double)m, &sn, &cs);
-    return mk((float)cs, (float)sn);
+    return phase_at(ph, (j * j) % (2 * m));
 }
 
 // a[j] = z[j] conj(c_j) for j < m, 0 for m <= j < p
 __global__ __launch_bounds__(256) void bluestein_pre_kernel(const cplx* __restrict__ z,
-                                                            cplx* __restrict__ a, long m, long p)
+                                                            cplx* __restrict__ a, long m, long p,
+                                                            PhaseTable ph)
 {
     const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= p) return;
-    a[j] = j < m ? cmul(z[j], conjg(chirp(j, m))) : mk(0.f, 0.f);
+    a[j] = j < m ? cmul(z[j], conjg(chirp(j, m, ph))) : mk(0.f, 0.f);
 }
 
 // b[j] = c_j for j < m, b[p-j] = c_j for 0 < j < m, zero elsewhere
-__global__ __launch_bounds__(256) void bluestein_kernel_kernel(cplx* __restrict__ b, long m, long p)
+__global__ __launch_bounds__(256) void bluestein_kernel_kernel(cplx* __restrict__ b, long m, long p,
+                                                              PhaseTable ph)
 {
     const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= p) return;
     cplx v = mk(0.f, 0.f);
-    if (j < m) v = chirp(j, m);
-    else if (j > p - m) v = chirp(p - j, m);
+    if (j < m) v = chirp(j, m, ph);
+    else if (j > p - m) v = chirp(p - j, m, ph);
     b[j] = v;
 }
 
@@ -403,11 +446,12 @@ __global__ __launch_bounds__(256) void pointwise_mul_kernel(cplx* __restrict__ a
 }
 
 __global__ __launch_bounds__(256) void bluestein_post_kernel(const cplx* __restrict__ y,
-                                                             cplx* __restrict__ Z, long m)
+                                                             cplx* __restrict__ Z, long m,
+                                                             PhaseTable ph)
 {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
-    Z[k] = cmul(y[k], conjg(chirp(k, m)));
+    Z[k] = cmul(y[k], conjg(chirp(k, m, ph)));
 }
 
 static unsigned blocks_for(long n) { return (unsigned)((n + 255) / 256); }
@@ -456,20 +500,24 @@ int fft_r2c_plan(const R2CPlan& rp, const float* in, cplx* out, cplx* work, hipS
         cplx* a = work + m;
         cplx* kb = a + p;
         cplx* scratch = kb + p;
-        hipLaunchKernelGGL(bluestein_kernel_kernel, dim3(blocks_for(p)), dim3(256), 0, s, kb, m, p);
+        PhaseTable ch;
+        FDAS_TRY(phase_table(2LL * m, &ch));
+        hipLaunchKernelGGL(bluestein_kernel_kernel, dim3(blocks_for(p)), dim3(256), 0, s, kb, m, p, ch);
         FDAS_TRY(check_launch("bluestein_kernel_kernel"));
         FDAS_TRY(fft_c2c(kb, kb, p, -1, scratch, s));
-        hipLaunchKernelGGL(bluestein_pre_kernel, dim3(blocks_for(p)), dim3(256), 0, s, z, a, m, p);
+        hipLaunchKernelGGL(bluestein_pre_kernel, dim3(blocks_for(p)), dim3(256), 0, s, z, a, m, p, ch);
         FDAS_TRY(check_launch("bluestein_pre_kernel"));
         FDAS_TRY(fft_c2c(a, a, p, -1, scratch, s));
         hipLaunchKernelGGL(pointwise_mul_kernel, dim3(blocks_for(p)), dim3(256), 0, s, a, kb, p,
                            (float)(1.0 / (double)p));
         FDAS_TRY(check_launch("pointwise_mul_kernel"));
         FDAS_TRY(fft_c2c(a, a, p, +1, scratch, s));
-        hipLaunchKernelGGL(bluestein_post_kernel, dim3(blocks_for(m)), dim3(256), 0, s, a, Z, m);
+        hipLaunchKernelGGL(bluestein_post_kernel, dim3(blocks_for(m)), dim3(256), 0, s, a, Z, m, ch);
         FDAS_TRY(check_launch("bluestein_post_kernel"));
     }
-    hipLaunchKernelGGL(r2c_split_kernel, dim3(blocks_for(m + 1)), dim3(256), 0, s, Z, out, m);
+    PhaseTable sp;
+    FDAS_TRY(phase_table(2LL * m, &sp));
+    hipLaunchKernelGGL(r2c_split_kernel, dim3(blocks_for(m + 1)), dim3(256), 0, s, Z, out, m, sp);
     return check_launch("r2c_split_kernel");
 }
 

@@ -337,12 +337,13 @@ void harmsum_variant_shape(int v, int* threads, int* rows, int* cols)
 
 int launch_harmsum(const void* d_fund, float* d_out, const void* d_sub, int prec, const HarmSumArgs& hs,
                    int step0, int nsteps, int Z0, const int* d_fullrlo, const int* d_rlo_h,
-                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s)
+                   fdas_candidate* d_cands, int* d_ncand, hipStream_t s, int variant)
 {
     if (nsteps <= 0) return FDAS_OK;
     const int st = d_out ? 1 : 0;
 #define HS_ARGS d_fund, d_out, d_sub, prec, hs, step0, nsteps, Z0, d_fullrlo, d_rlo_h, d_cands, d_ncand, st, s
-    switch (g_hs_variant.load()) {
+    if (variant < 0 || variant >= kHsCount) variant = g_hs_variant.load();
+    switch (variant) {
         case 1: return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
         case 2: return harmsum_t<128, 4, 896, 4608>(HS_ARGS);
         case 3: return harmsum_t<128, 3, 1024, 4160>(HS_ARGS);

@@ -181,6 +181,10 @@ struct fdas_plan {
     int* d_ncand = nullptr;
     int cand_cap = 0;
     HarmSumArgs hs{};
+    // launch shapes in effect when the plan was made (process-wide choices made
+    // later, by the autotuner or the setters, apply to later plans only)
+    int corr_var[MAX_HARM]{};
+    int hs_var = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
     DevAlloc* al = nullptr;         // source of the plan-owned buffers
@@ -545,6 +549,8 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     }
     for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
     hs.cand_cap = pl->cand_cap;
+    for (int h = 0; h < H; ++h) pl->corr_var[h] = corr_variant(pl->L[h]);
+    pl->hs_var = harmsum_variant();
     FDAS_HIP(hipStreamSynchronize(pl->stream));
     return FDAS_OK;
 }
@@ -614,7 +620,7 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         for (int h = 1; h < pl->H; ++h) {
             TimedLaunch tl(pl, FDAS_K_CORR_SUB);
             char* sub_h = static_cast<char*>(pl->d_sub) + (size_t)pl->hs.sub_off[h] * pl->pbytes;
-            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, sub_h, pl->W[h]), pl->stream));
+            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, sub_h, pl->W[h]), pl->stream, pl->corr_var[h]));
         }
         // summed powers for the caller, FP32; with FP32 planes the fundamental
         // plane is that buffer itself and the sum goes back in place
@@ -622,7 +628,7 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         void* plane = out && pl->prec == FDAS_PREC_FP32 ? static_cast<void*>(out) : pl->d_plane;
         {
             TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-            FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream));
+            FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream, pl->corr_var[0]));
         }
         // The harmonic sum as its own tiled kernel: summing inside the fundamental
         // correlation's epilogue (gathering sub-harmonic cells after each inverse
@@ -631,7 +637,7 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         // consumer is the threshold, which the kernel applies to every cell.
         TimedLaunch tl(pl, FDAS_K_HARMSUM);
         FDAS_TRY(launch_harmsum(plane, out, pl->d_sub, pl->prec, pl->hs, c0, cs, pl->Z0, pl->d_fullrlo,
-                                pl->d_rlo_h, pl->d_cands, pl->d_ncand, pl->stream));
+                                pl->d_rlo_h, pl->d_cands, pl->d_ncand, pl->stream, pl->hs_var));
     }
     return FDAS_OK;
 }
@@ -671,6 +677,20 @@ int plan_create_ex(const fdas_params* params, int ntrials, const long* lengths, 
 
 // Shared handle on a plan's template banks (keeps them resident while held).
 std::shared_ptr<void> plan_banks(fdas_plan* pl) { return pl->banks; }
+
+int plan_set_harmsum_variant(fdas_plan* pl, int v)
+{
+    if (v < 0 || v >= harmsum_variant_count()) {
+        set_error("no harmonic-sum variant %d", v);
+        return FDAS_ERR_INVALID;
+    }
+    if (pl->graph) {
+        (void)hipGraphExecDestroy(pl->graph);
+        pl->graph = nullptr;
+    }
+    pl->hs_var = v;
+    return FDAS_OK;
+}
 
 }  // namespace fdas
 

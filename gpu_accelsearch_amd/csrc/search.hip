@@ -429,7 +429,7 @@ __global__ __launch_bounds__(T) void corr_split_kernel(
 
 // Split bank rows (see corr_split_kernel) from full rows of length L.
 __global__ __launch_bounds__(256) void bank_split_kernel(const cplx* __restrict__ in, int L,
-                                                         cplx* __restrict__ out)
+                                                         cplx* __restrict__ out, PhaseTable ph)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int z = blockIdx.y;
@@ -437,11 +437,9 @@ __global__ __launch_bounds__(256) void bank_split_kernel(const cplx* __restrict_
     if (k >= LH) return;
     const cplx* row = in + (size_t)z * L;
     const cplx lo = mk(row[k].x, -row[k].y), hi = mk(row[k + LH].x, -row[k + LH].y);
-    double sw, cw;
-    sincospi(2.0 * (double)k / (double)L, &sw, &cw);
     const cplx d = mk(lo.x - hi.x, lo.y - hi.y);
     out[(size_t)z * L + k] = mk(lo.x + hi.x, lo.y + hi.y);
-    out[(size_t)z * L + LH + k] = cmul(d, mk((float)cw, (float)sw));
+    out[(size_t)z * L + LH + k] = cmul(d, phase_at(ph, k));   // exp(2 pi i k / L)
 }
 
 // Reduced-precision copy of a template bank: per row the exponent and the packed
@@ -827,7 +825,9 @@ int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, in
 
 int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t s)
 {
-    hipLaunchKernelGGL(bank_split_kernel, dim3((L / 2 + 255) / 256, Z), dim3(256), 0, s, d_bank, L, d_out);
+    PhaseTable ph;
+    FDAS_TRY(phase_table(L, &ph));
+    hipLaunchKernelGGL(bank_split_kernel, dim3((L / 2 + 255) / 256, Z), dim3(256), 0, s, d_bank, L, d_out, ph);
     return check_launch("bank_split_kernel");
 }
 

@@ -177,7 +177,7 @@ struct OptTask {
     int numharm;
     double r, z;          // first-harmonic start point
     double dr, dz;        // initial grid spacing (first-harmonic units)
-    float norm[MAX_HARM]; // 1 / (local median power / ln 2) per harmonic
+    float norm[MAX_HARM]; // 1 / (local median power / ln 2) per harmonic (the kernel's)
 };
 
 struct OptResult {
@@ -211,6 +211,38 @@ __device__ float harm_power(const cplx* __restrict__ spec, int nbins, double r, 
     return (float)((ar * ar + ai * ai) * norm);
 }
 
+constexpr int OPT_WIN = 256;   // bins of the local power estimate per harmonic
+
+// Local normalisation of harmonic k (1-based) of a candidate at first-harmonic
+// r: ln 2 / (lower median of |a|^2 over the OPT_WIN bins around k r, clamped to
+// the spectrum), 0 for a zero median.  Exact selection by rank counting in LDS.
+__device__ float local_norm(const cplx* __restrict__ spec, int nbins, double r, int k, float* pw,
+                            float* med, int t)
+{
+    const int wn = min(OPT_WIN, nbins);
+    long lo = (long)floor(k * r) - wn / 2;
+    lo = max(0L, min(lo, (long)nbins - wn));
+    __syncthreads();   // earlier readers of pw are done
+    if (t < wn) {
+        const cplx a = spec[lo + t];
+        pw[t] = a.x * a.x + a.y * a.y;
+    }
+    __syncthreads();
+    const int rank = (wn - 1) / 2;
+    if (t < wn) {
+        const float v = pw[t];
+        int lt = 0, le = 0;
+        for (int q = 0; q < wn; ++q) {
+            lt += pw[q] < v;
+            le += pw[q] <= v;
+        }
+        if (lt <= rank && rank < le) *med = v;   // every such thread holds the same value
+    }
+    __syncthreads();
+    const float m = *med;
+    return m > 0.f ? (float)(kLn2 / (double)m) : 0.f;
+}
+
 __global__ __launch_bounds__(256) void optimize_kernel(const cplx* __restrict__ spec_all,
                                                        const OptTask* __restrict__ tasks,
                                                        OptResult* __restrict__ out)
@@ -218,9 +250,18 @@ __global__ __launch_bounds__(256) void optimize_kernel(const cplx* __restrict__ 
     __shared__ float sum[OPT_G * OPT_G];
     __shared__ double cen[2];
     __shared__ int best;
-    const OptTask tk = tasks[blockIdx.x];
+    __shared__ float pw[OPT_WIN];
+    __shared__ float med;
+    __shared__ float norm[MAX_HARM];
+    OptTask tk = tasks[blockIdx.x];
     const cplx* spec = spec_all + tk.spec_off;
     const int t = threadIdx.x;
+    for (int k = 1; k <= tk.numharm; ++k) {
+        const float nk = local_norm(spec, tk.nbins, tk.r, k, pw, &med, t);
+        if (t == 0) norm[k - 1] = nk;
+    }
+    __syncthreads();
+    for (int k = 0; k < tk.numharm; ++k) tk.norm[k] = norm[k];
     if (t == 0) {
         cen[0] = tk.r;
         cen[1] = tk.z;
@@ -260,22 +301,13 @@ __global__ __launch_bounds__(256) void optimize_kernel(const cplx* __restrict__ 
     }
 }
 
-float lower_median(std::vector<float>& v)
-{
-    const size_t k = (v.size() - 1) / 2;
-    std::nth_element(v.begin(), v.begin() + k, v.end());
-    return v[k];
-}
-
 }  // namespace
 
 int optimize_candidates(const cplx* d_spec, const std::vector<long long>& spec_off,
                         const std::vector<int>& nbins, fdas_accelcand* c, int n, hipStream_t s)
 {
     if (n <= 0) return FDAS_OK;
-    constexpr int WIN = 256;   // bins of the local power estimate per harmonic
     std::vector<OptTask> tasks(n);
-    std::vector<cplx> win(WIN);
     for (int i = 0; i < n; ++i) {
         OptTask& tk = tasks[i];
         const int tr = c[i].trial;
@@ -287,18 +319,7 @@ int optimize_candidates(const cplx* d_spec, const std::vector<long long>& spec_o
         // one search-grid cell of the summed column, in first-harmonic units
         tk.dr = 0.5 * DR / c[i].numharm;
         tk.dz = 0.5 * DZ / c[i].numharm;
-        const int wn = std::min(WIN, tk.nbins);
-        for (int k = 1; k <= c[i].numharm; ++k) {
-            long lo = (long)std::floor(k * c[i].r) - wn / 2;
-            lo = std::max(0L, std::min(lo, (long)tk.nbins - wn));
-            FDAS_HIP(hipMemcpyAsync(win.data(), d_spec + tk.spec_off + lo, wn * sizeof(cplx),
-                                    hipMemcpyDeviceToHost, s));
-            FDAS_HIP(hipStreamSynchronize(s));
-            std::vector<float> pw(wn);
-            for (int q = 0; q < wn; ++q) pw[q] = win[q].x * win[q].x + win[q].y * win[q].y;
-            const double med = lower_median(pw);
-            tk.norm[k - 1] = med > 0.0 ? (float)(kLn2 / med) : 0.f;
-        }
+        for (int k = 0; k < MAX_HARM; ++k) tk.norm[k] = 0.f;   // set on the device
     }
     OptTask* d_tasks = nullptr;
     OptResult* d_out = nullptr;

@@ -180,8 +180,10 @@ int time_corr_variants(int L, int prec, hipStream_t s, std::vector<double>* us)
     return rc;
 }
 
-// Milliseconds per DM trial (R2C, whitening and every step) at one fftlen.
-int time_trial(const fdas_params& base, int fftlen, hipStream_t s, double* ms_out)
+// Milliseconds per DM trial (R2C, whitening and every step) at one fftlen, with
+// harmonic-sum tile shape hs_variant (< 0: the process-wide one).
+int time_trial(const fdas_params& base, int fftlen, hipStream_t s, double* ms_out,
+               int hs_variant = -1)
 {
     fdas_params p = base;
     p.fftlen = fftlen;
@@ -189,10 +191,12 @@ int time_trial(const fdas_params& base, int fftlen, hipStream_t s, double* ms_ou
     const long n = base.n;
     fdas_plan* pl = nullptr;
     FDAS_TRY(plan_create_ex(&p, 1, &n, true, s, nullptr, &pl));
-    int rc = FDAS_OK;
+    int rc = hs_variant >= 0 ? plan_set_harmsum_variant(pl, hs_variant) : FDAS_OK;
     float* x = fdas_batch_series(pl, 0);
-    hipLaunchKernelGGL(fill_noise_kernel, dim3(blocks(n)), dim3(256), 0, s, x, n, 7u);
-    rc = check_launch("fill_noise_kernel");
+    if (rc == FDAS_OK) {
+        hipLaunchKernelGGL(fill_noise_kernel, dim3(blocks(n)), dim3(256), 0, s, x, n, 7u);
+        rc = check_launch("fill_noise_kernel");
+    }
     hipEvent_t a = nullptr, b = nullptr;
     if (rc == FDAS_OK && (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)) {
         set_error("autotune: event creation failed");
@@ -322,9 +326,8 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
         int bestv = incumbent;
         bool timed = false;
         for (int v = 0; v < harmsum_variant_count() && v < FDAS_TUNE_MAXV; ++v) {
-            set_harmsum_variant(v);
             double ms = 0.0;
-            const int r = time_trial(*base, cand[i], s, &ms);
+            const int r = time_trial(*base, cand[i], s, &ms, v);
             if (r != FDAS_OK) {
                 if (r != FDAS_ERR_INVALID && r != FDAS_ERR_UNSUPPORTED) rc = r;
                 break;

@@ -399,6 +399,10 @@ typedef struct fdas_tune_report {
 
 int fdas_autotune(const fdas_params *base, const int *fftlens, int nfftlen,
                   fdas_tune_report *report);
+/* The selections below (and the autotuner's) apply to plans created afterwards:
+ * a plan keeps the launch shapes in effect when it was created, so a tuning run
+ * never changes the kernels, and hence the rounding, of a plan or server request
+ * already running. */
 int fdas_corr_variant_count(int fftlen);
 int fdas_harmsum_variant_count(void);
 int fdas_get_harmsum_variant(void);

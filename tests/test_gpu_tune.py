@@ -1,6 +1,7 @@
 """Autotuner: every correlation launch shape agrees with the oracle (1e-4 of
 max(P_ref, 1)) and, in reduced-precision storage, with the default shape; the
-tuner's choices apply to later plans."""
+tuner's choices apply to later plans, and a plan keeps the shapes in effect when
+it was made."""
 import numpy as np
 import pytest
 
@@ -36,17 +37,19 @@ def test_every_corr_variant_matches_oracle(fd, oracle, n, zmax, numharm, fftlen)
     x = synth.plain_series(n, seed=21, r0=30000.3, z0=zmax / 2, power=80.0)
     ref = oracle.Search(x, n, 6.4e-5, zmax, numharm, fftlen)
     L = fd.lib()
+    P = fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen)
     try:
-        with fd.AccelSearch(fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen)) as s:
+        with fd.AccelSearch(P) as s:
             lens = sorted({hm[2] for hm in s.harmonics})
-            nv = max(L.fdas_corr_variant_count(l) for l in lens)
             steps = [0, s.nsteps // 2]
-            want = [ref.step_plane(st) for st in steps]
-            s.prepare(x)
-            for v in range(nv):
-                for l in lens:
-                    assert L.fdas_set_corr_variant(l, min(v, L.fdas_corr_variant_count(l) - 1)) == 0
-                print_case("corr variant", v, "lengths", lens)
+        nv = max(L.fdas_corr_variant_count(l) for l in lens)
+        want = [ref.step_plane(st) for st in steps]
+        for v in range(nv):
+            for l in lens:
+                assert L.fdas_set_corr_variant(l, min(v, L.fdas_corr_variant_count(l) - 1)) == 0
+            print_case("corr variant", v, "lengths", lens)
+            with fd.AccelSearch(P) as s:   # a plan takes the shapes in effect at creation
+                s.prepare(x)
                 for st, w in zip(steps, want):
                     got = s.run_steps(st, 1)[0]
                     e = float(np.max(np.abs(got - w) / np.maximum(w, 1.0)))
@@ -90,23 +93,53 @@ def test_every_harmsum_shape_is_bit_identical(fd):
     ref = None
     default = L.fdas_get_harmsum_variant()
     try:
-        with fd.AccelSearch(P) as s:
-            s.prepare(x)
-            for v in range(L.fdas_harmsum_variant_count()):
-                assert L.fdas_set_harmsum_variant(v) == 0
-                print_case("harmsum shape", v)
+        for v in range(L.fdas_harmsum_variant_count()):
+            assert L.fdas_set_harmsum_variant(v) == 0
+            print_case("harmsum shape", v)
+            with fd.AccelSearch(P) as s:
+                s.prepare(x)
                 planes = s.run_steps(0, s.nsteps)
                 s.search(x)
                 cands = s.candidates()
-                key = (planes.view(np.uint32), sorted(map(tuple, cands.tolist())))
-                if ref is None:
-                    ref = key
-                    assert len(key[1]) > 0
-                else:
-                    np.testing.assert_array_equal(key[0], ref[0])
-                    assert key[1] == ref[1]
+            key = (planes.view(np.uint32), sorted(map(tuple, cands.tolist())))
+            if ref is None:
+                ref = key
+                assert len(key[1]) > 0
+            else:
+                np.testing.assert_array_equal(key[0], ref[0])
+                assert key[1] == ref[1]
     finally:
         L.fdas_set_harmsum_variant(default)
+
+
+def test_plan_keeps_its_launch_shapes(fd):
+    """Changing the process-wide shapes leaves a plan's kernels (and so its bits)
+    as they were when it was made; a plan made afterwards takes the new shapes."""
+    from gpu_accelsearch_amd import synth
+    n = 1 << 20
+    x = synth.plain_series(n, seed=29, r0=21000.3, z0=8.0, power=80.0)
+    L = fd.lib()
+    P = fd.SearchParams(n=n, zmax=200, numharm=8, fftlen=8192)
+    print_case("plan keeps its shapes")
+    try:
+        L.fdas_set_corr_variant(8192, 0)
+        L.fdas_set_harmsum_variant(0)
+        with fd.AccelSearch(P) as a:
+            a.prepare(x)
+            before = a.run_steps(0, 2).copy()
+            L.fdas_set_corr_variant(8192, 4)     # 1024 threads x 8 values
+            L.fdas_set_harmsum_variant(6)
+            after = a.run_steps(0, 2)
+            np.testing.assert_array_equal(before.view(np.uint32), after.view(np.uint32))
+            with fd.AccelSearch(P) as b:
+                b.prepare(x)
+                other = b.run_steps(0, 2)
+            # a different transform shape rounds differently in FP32
+            assert not np.array_equal(before.view(np.uint32), other.view(np.uint32))
+            assert float(np.max(np.abs(other - before) / np.maximum(before, 1.0))) <= 1e-5
+    finally:
+        L.fdas_set_corr_variant(8192, 0)
+        L.fdas_set_harmsum_variant(0)
 
 
 # Oracle bounds of the reduced-precision storage modes: the 1e-3 tolerance for
@@ -133,25 +166,28 @@ def test_every_corr_variant_reduced_precision(fd, oracle, precision, fftlen):
         P = fd.SearchParams(n=n, zmax=200, numharm=8, fftlen=fftlen, precision=precision)
         with fd.AccelSearch(P) as s:
             lens = sorted({hm[2] for hm in s.harmonics})
-            shapes = {l: fd.corr_variant_shapes(l) for l in lens}
-            nv = max(len(v) for v in shapes.values())
             steps = [0, s.nsteps // 2]
-            want = [ref.step_plane(st) for st in steps]
+        shapes = {l: fd.corr_variant_shapes(l) for l in lens}
+        nv = max(len(v) for v in shapes.values())
+        want = [ref.step_plane(st) for st in steps]
+        for l in lens:
+            L.fdas_set_corr_variant(l, 0)
+        with fd.AccelSearch(P) as s:
             s.prepare(x)
-            for l in lens:
-                L.fdas_set_corr_variant(l, 0)
             base = [s.run_steps(st, 1)[0].copy() for st in steps]
-            for b, w in zip(base, want):
-                e = float(np.max(np.abs(b - w) / np.maximum(w, 1.0)))
-                print(f"TUNE {precision} {fftlen} default shape oracle error {e:.3e}")
-                assert e <= PREC_TOL[precision]
-            for v in range(1, nv):
-                split = False
-                for l in lens:
-                    vl = min(v, len(shapes[l]) - 1)
-                    split |= shapes[l][vl]["split"]
-                    assert L.fdas_set_corr_variant(l, vl) == 0
-                print_case("corr variant", v, precision, "lengths", lens, "split" if split else "")
+        for b, w in zip(base, want):
+            e = float(np.max(np.abs(b - w) / np.maximum(w, 1.0)))
+            print(f"TUNE {precision} {fftlen} default shape oracle error {e:.3e}")
+            assert e <= PREC_TOL[precision]
+        for v in range(1, nv):
+            split = False
+            for l in lens:
+                vl = min(v, len(shapes[l]) - 1)
+                split |= shapes[l][vl]["split"]
+                assert L.fdas_set_corr_variant(l, vl) == 0
+            print_case("corr variant", v, precision, "lengths", lens, "split" if split else "")
+            with fd.AccelSearch(P) as s:
+                s.prepare(x)
                 for st, b, w in zip(steps, base, want):
                     got = s.run_steps(st, 1)[0]
                     if split:
