@@ -275,7 +275,8 @@ def serving_form(count, rate, sweep, sweep_count, seed=6):
         main = serving_run(srv, pool, synth.serving_stream(count, rate, seed=seed), rate)
         runs = []
         for k, r in enumerate(sweep):
-            s = serving_run(srv, pool, synth.serving_stream(sweep_count, r, seed=seed + 10 + k), r)
+            # the same request mix at every rate (arrival times scale with 1/rate)
+            s = serving_run(srv, pool, synth.serving_stream(sweep_count, r, seed=seed + 10), r)
             runs.append({key: s[key] for key in ("offered_rate_per_s", "requests", "wall_s",
                                                  "dm_trials_per_s", "p50_ms", "p99_ms",
                                                  "max_active_seen", "failed")})

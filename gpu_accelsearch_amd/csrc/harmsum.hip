@@ -84,8 +84,9 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
     // fundamental powers, loaded from clamped (always valid) cells without a
     // branch so that every load is in flight at once; they are decoded and
     // tested once the first round's loads are in flight too, so both arrive
-    // behind one memory wait (cells outside the tile are never stored or tested)
-    float acc[HS_RJ][HS_CPT], p0[HS_RJ][HS_CPT];
+    // behind one memory wait.  Cells outside the tile start at -inf, so no stage
+    // sum of theirs passes a cut, and they are never stored.
+    float acc[HS_RJ][HS_CPT];
     PE raw[HS_RJ][HS_CPT];
     unsigned hit = 0;
     bool first = true;
@@ -96,17 +97,31 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
             const int col = min(c0 + t + k * HS_T, c0 + ncol - 1);
             raw[r][k] = base_row[(size_t)min(r, nrow - 1) * U + col];
         }
+    // Stage test: one max over the thread's cells; only a thread whose max passes
+    // the cut (rare) marks its cells one by one.
+    auto mark_cells = [&](float pc) {
+        float m = acc[0][0];
+#pragma unroll
+        for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+            for (int k = 0; k < HS_CPT; ++k) m = fmaxf(m, acc[r][k]);
+        if (m > pc) {
+#pragma unroll
+            for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+                for (int k = 0; k < HS_CPT; ++k)
+                    if (acc[r][k] > pc) hit |= 1u << (r * HS_CPT + k);
+        }
+    };
     auto mark_fundamental = [&]() {
 #pragma unroll
         for (int r = 0; r < HS_RJ; ++r)
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
-                acc[r][k] = PStore<PE>::get(raw[r][k]);
-                p0[r][k] = acc[r][k];
-                const int col = c0 + t + k * HS_T;
-                if (r < nrow && col < c0 + ncol && acc[r][k] > hs.powcut[0])
-                    hit |= 1u << (r * HS_CPT + k);
+                const bool ok = r < nrow && t + k * HS_T < ncol;
+                acc[r][k] = ok ? PStore<PE>::get(raw[r][k]) : -INFINITY;
             }
+        mark_cells(hs.powcut[0]);
     };
     if (hs.H <= 1) mark_fundamental();
 
@@ -192,55 +207,38 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
         for (int g = h; g < h2; ++g) {
             const int ri0a = u(geo[g][0]), nca = u(geo[g][1]), base = u(geo[g][6 + HS_RJ]);
             const int c = u(geo[g][3]), lm = u(geo[g][4]), r2 = u(geo[g][5]);
+            // staged column of each of the thread's columns: hs_rind with the
+            // product q c stepped by HS_T c from column to column, clamped to the
+            // tile's last column (cells beyond it are -inf and never used)
+            const int ri_last = u(hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0a);
+            unsigned qc = (q0 + (unsigned)(c0 + t)) * (unsigned)c + (1u << (lm - 1));
+            const unsigned qstep = (unsigned)(HS_T * c);
             int ri[HS_CPT];
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
-                const int col = min(c0 + t + k * HS_T, c0 + ncol - 1);
-                ri[k] = hs_rind(q0 + col, c, lm, r2) - ri0a;
+                ri[k] = base + min((int)(qc >> lm) - (r2 + ri0a), ri_last);
+                qc += qstep;
             }
 #pragma unroll
             for (int r = 0; r < HS_RJ; ++r) {
                 const int zi = u(geo[g][6 + r]);
 #pragma unroll
-                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[base + zi * nca + ri[k]];
+                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[zi * nca + ri[k]];
             }
-            if (((g + 1) & g) == 0) {
-                ++stage;
-                const float pc = hs.powcut[stage];
-#pragma unroll
-                for (int r = 0; r < HS_RJ; ++r)
-#pragma unroll
-                    for (int k = 0; k < HS_CPT; ++k)
-                        if (r < nrow && c0 + t + k * HS_T < c0 + ncol && acc[r][k] > pc)
-                            hit |= 1u << (r * HS_CPT + k);
-            }
+            if (((g + 1) & g) == 0) mark_cells(hs.powcut[++stage]);
         }
     }
 
-    if (store) {
-        float* out_row = out + ((size_t)s * Z0 + j0) * U;
-#pragma unroll
-        for (int r = 0; r < HS_RJ; ++r)
-#pragma unroll
-            for (int k = 0; k < HS_CPT; ++k) {
-                const int col = c0 + t + k * HS_T;
-                if (r < nrow && col < c0 + ncol) out_row[(size_t)r * U + col] = acc[r][k];
-            }
-    }
-
     // rare path: stage sums of the marked cells, re-accumulated in the same order
+    // (before the store, which may overwrite the fundamental plane in place)
     if (hit) {
         for (int b = 0; b < HS_RJ * HS_CPT; ++b) {
             if (!((hit >> b) & 1u)) continue;
             const int r = b / HS_CPT, k = b % HS_CPT;
             const int j = j0 + r, col = c0 + t + k * HS_T;
             const int zz = hs.zlo0 + DZ * j;
-            float a2 = 0.f;
-#pragma unroll
-            for (int rr = 0; rr < HS_RJ; ++rr)
-#pragma unroll
-                for (int kk = 0; kk < HS_CPT; ++kk)
-                    if (rr == r && kk == k) a2 = p0[rr][kk];
+            // the fundamental power, read back (the cell is stored only below)
+            float a2 = PStore<PE>::get(base_row[(size_t)r * U + col]);
             int st = 0;
             if (a2 > hs.powcut[0]) hs_emit(cands, ncand, hs.cand_cap, gstep, j, col, 0, a2);
             for (int h = 1; h < hs.H; ++h) {
@@ -254,6 +252,17 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
                 }
             }
         }
+    }
+
+    if (store) {
+        float* out_row = out + ((size_t)s * Z0 + j0) * U;
+#pragma unroll
+        for (int r = 0; r < HS_RJ; ++r)
+#pragma unroll
+            for (int k = 0; k < HS_CPT; ++k) {
+                const int col = c0 + t + k * HS_T;
+                if (r < nrow && col < c0 + ncol) out_row[(size_t)r * U + col] = acc[r][k];
+            }
     }
 }
 
