@@ -119,7 +119,7 @@ class FdasInf(C.Structure):
 
 
 TUNE_MAX = 12
-TUNE_MAXV = 8
+TUNE_MAXV = 16
 
 
 class FdasTuneReport(C.Structure):

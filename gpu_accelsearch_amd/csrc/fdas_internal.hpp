@@ -38,6 +38,12 @@ bool fft_row_supported(long n);
 const cplx* fft_twiddles(int n, int* err);
 // Same for a transform run by t threads (the pass radices depend on n/t).
 const cplx* fft_twiddles_t(int n, int t, int* err);
+// Table of the wavefront-split transform of length n run by t threads (WaveFFT in
+// fft_block.hpp; cached per device).
+const cplx* fft_wave_twiddles(int n, int t, int* err);
+// Rows of length L into the wavefront-decimated order out[row][(n mod W) L/W + n/W]
+// = in[row][n], W = 2^lw (not in place).
+int decimate_rows(const cplx* in, cplx* out, int L, int lw, long nrows, hipStream_t s);
 // Radix-r step table of the length-r*m transforms (r = 3, 5, 7):
 // w^(q k) at (r-1) k + q - 1, w = exp(2 pi i / (r m)), q in [1, r), k in [0, m)
 // (cached per device).
@@ -175,6 +181,8 @@ struct SegPrepArgs {
     int* seg_exp;
     int eslot_base;
     int prec;
+    int decim;   // log2 W: store the segment in the wavefront-decimated order
+                 // a'[(j mod W) (L/2)/W + j/W] = a[j] (0: natural order)
 };
 int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s);
 
@@ -206,6 +214,9 @@ int corr_variant(int L);
 int set_corr_variant(int L, int v);
 void corr_variant_shape(int L, int v, int* threads, int* steps, int* split = nullptr,
                         int* form = nullptr);
+// log2 of the wavefront decimation a correlation variant reads its segment and
+// template rows in (0: natural order).
+int corr_variant_decim(int L, int v);
 // Split-form rows of a full FP32 bank (Z rows of length L).
 int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t s);
 // Reduced-precision copy of a bank of Z rows of length L.

@@ -118,6 +118,74 @@ const cplx* fft_twiddles_t(int n, int t, int* err)
     return d;
 }
 
+template <int N, int T>
+static std::vector<cplx> host_wave_twiddles()
+{
+    using F = WaveFFT<N, T, 1>;
+    std::vector<cplx> tw(F::TW_COUNT);
+    F::build_twiddles(tw.data());
+    return tw;
+}
+
+const cplx* fft_wave_twiddles(int n, int t, int* err)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, cplx*> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        set_error("hipGetDevice failed");
+        *err = FDAS_ERR_HIP;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    auto key = std::make_pair(dev, n * 4096 + t);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    std::vector<cplx> h;
+    if (n == 8192 && t == 512) h = host_wave_twiddles<8192, 512>();
+    else if (n == 4096 && t == 512) h = host_wave_twiddles<4096, 512>();
+    else if (n == 4096 && t == 256) h = host_wave_twiddles<4096, 256>();
+    else if (n == 2048 && t == 128) h = host_wave_twiddles<2048, 128>();
+    if (h.empty()) {
+        set_error("no wavefront-split transform of length %d with %d threads", n, t);
+        *err = FDAS_ERR_UNSUPPORTED;
+        return nullptr;
+    }
+    cplx* d = nullptr;
+    hipError_t e = hipMalloc(&d, h.size() * sizeof(cplx));
+    if (e == hipSuccess) e = hipMemcpy(d, h.data(), h.size() * sizeof(cplx), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        *err = hip_fail(e, "twiddle table", __FILE__, __LINE__);
+        return nullptr;
+    }
+    cache[key] = d;
+    return d;
+}
+
+// Rows of length L into the wavefront-decimated order: out[row][(n mod W) L/W + n/W]
+// = in[row][n], W = 2^lw (templates of the wavefront-split correlation).
+__global__ __launch_bounds__(256) void decimate_rows_kernel(const cplx* __restrict__ in,
+                                                            cplx* __restrict__ out, int L, int lw)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= L) return;
+    const size_t row = (size_t)blockIdx.y * L;
+    const int W = 1 << lw;
+    out[row + (size_t)(n & (W - 1)) * (L >> lw) + (n >> lw)] = in[row + n];
+}
+
+int decimate_rows(const cplx* in, cplx* out, int L, int lw, long nrows, hipStream_t s)
+{
+    if (nrows <= 0 || lw <= 0) return FDAS_OK;
+    if (nrows > 65535) {
+        set_error("decimate_rows: %ld rows exceed the grid limit", nrows);
+        return FDAS_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(decimate_rows_kernel, dim3((L + 255) / 256, (unsigned)nrows), dim3(256), 0, s,
+                       in, out, L, lw);
+    return check_launch("decimate_rows_kernel");
+}
+
 const cplx* fft_odd_twiddles(int r, int m, int* err)
 {
     static std::mutex mu;

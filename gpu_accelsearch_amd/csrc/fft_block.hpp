@@ -45,9 +45,21 @@ namespace fdas {
 
 #ifdef __HIPCC__
 __device__ __forceinline__ void block_sync() { __syncthreads(); }
+// Orders one wavefront's LDS accesses: the wavefront's lanes execute together and
+// its LDS operations complete in issue order, so an exchange among the lanes of
+// one wavefront only needs the compiler to keep every access on its side of the
+// point (release and acquire fences at wavefront scope around a wave barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 #elif defined(FDAS_HOST_BARRIER)
-// A host test that runs each workgroup thread as an OS thread supplies the barrier.
+// A host test that runs each workgroup thread as an OS thread supplies the barriers
+// (wave_sync: a barrier among the 64 threads of one wavefront).
 void host_barrier();
+void host_wave_barrier(int t);
 inline void block_sync() { host_barrier(); }
 #else
 inline void block_sync() {}
@@ -266,6 +278,33 @@ struct BlockFFT {
     // Input position expected in v[i] before pass 0.
     FDAS_HD static int in_index(int t, int i) { return t + i * T; }
 
+    // Passes P..NPASS-1 of a transform run by one wavefront (T == 64) through an
+    // LDS region of its own: the exchanges are ordered within the wavefront only.
+    template <int P>
+    FDAS_HD static void passes_from_wave(cplx* v, cplx* lds, const cplx* tw, int t)
+    {
+        static_assert(T == 64, "a wavefront transform has 64 threads");
+        if constexpr (P < NPASS) {
+            wave_barrier(t);
+            store<P - 1>(v, lds, t);
+            wave_barrier(t);
+            load<P>(v, lds, t);
+            compute<P>(v, t, tw);
+            passes_from_wave<P + 1>(v, lds, tw, t);
+        }
+    }
+    FDAS_HD static void wave_barrier(int t)
+    {
+#ifdef __HIPCC__
+        (void)t;
+        wave_sync();
+#elif defined(FDAS_HOST_BARRIER)
+        host_wave_barrier(t);
+#else
+        (void)t;
+#endif
+    }
+
     // Passes P..NPASS-1 (each preceded by an LDS exchange).
     template <int P>
     FDAS_HD static void passes_from(cplx* v, cplx* lds, const cplx* tw, int t)
@@ -399,6 +438,92 @@ struct BlockFFT {
 #pragma unroll
         for (int a = 0; a < NA; ++a) compute<0>(v[a], t, tw);
         passes_multi<NA, 1>(v, lds, tw, t);
+    }
+};
+
+// ------------------------------------------------------------------ wavefront split
+// A length-N transform by T threads (W = T/64 wavefronts, E = N/T values each),
+// decimated in time across the wavefronts.  With n = w + W m (w < W, m < NW =
+// N/W) and k = k2 + NW k1 (k2 < NW, k1 < W), w = exp(DIR 2 pi i / N):
+//   X[k2 + NW k1] = sum_w exp(DIR 2 pi i w k1 / W) (w^(w k2) F_w[k2]),
+//   F_w = DFT_NW(x[w + W m], m < NW).
+// Wavefront w computes F_w as a Stockham transform of its own (64 lanes x E
+// values), whose exchanges go through an LDS band of its own and are ordered
+// within the wavefront only, so one wavefront's LDS traffic overlaps the other
+// wavefronts' arithmetic.  Then the twiddle w^(w k2), one exchange across the
+// wavefronts (one workgroup barrier) that gives thread t the W values F_w[k2] of
+// each of its columns k2 = t + T b (b < E/W), and a radix-W DFT in registers.
+// Inputs come in the decimated order: v[i] holds x[in_index(t, i)] =
+// x[w + W (lane + 64 i)], contiguous over the lanes once the input is stored as
+// x'[w NW + m] = x[w + W m]; outputs come in natural order, contiguous over t.
+template <int N, int T, int DIR>
+struct WaveFFT {
+    static constexpr int W = T / 64, E = N / T, NW = N / W, NB = E / W;
+    using FW = BlockFFT<NW, 64, DIR>;
+    static_assert(T % 64 == 0 && W >= 2 && FW::E == E && E % W == 0, "wavefront split shape");
+    static constexpr int BAND = FW::LDS_ELEMS;      // one wavefront's region
+    static constexpr int SET = W * BAND;            // the workgroup's regions
+    // twiddle table: FW's inter-pass table, then exp(+2 pi i j / N), j < N
+    static constexpr int TW_CROSS = FW::TW_COUNT;
+    static constexpr int TW_COUNT = FW::TW_COUNT + N;
+
+    FDAS_HD static int in_index(int t, int i) { return (t >> 6) + W * FW::in_index(t & 63, i); }
+    FDAS_HD static int out_index(int t, int i) { return t + T * (i / W) + NW * (i % W); }
+
+    static void build_twiddles(cplx* tw)
+    {
+        FW::build_twiddles(tw);
+        for (int j = 0; j < N; ++j) {
+            const double a = 6.283185307179586476925286766559 * (double)j / (double)N;
+            tw[TW_CROSS + j] = mk((float)cos(a), (float)sin(a));
+        }
+    }
+
+    // Wavefront part: F_w of the values in v (see in_index), left in v at
+    // FW::out_index(lane, i), through band w of set.
+    FDAS_HD static void wave_part(cplx* v, cplx* set, const cplx* tw, int t)
+    {
+        const int w = t >> 6, lane = t & 63;
+        FW::template compute<0>(v, lane, tw);
+        FW::template passes_from_wave<1>(v, set + w * BAND, tw, lane);
+    }
+    // Cross part: twiddles, the exchange across wavefronts, radix-W DFTs.  The
+    // exchange reuses band w of set for F_w (after the wavefront's own passes);
+    // the caller alternates between two sets from call to call, so one barrier
+    // per call suffices: a set is written again only two calls later, after the
+    // barrier of the call in between, which every thread reaches after its loads.
+    FDAS_HD static void cross_store(const cplx* v, cplx* set, const cplx* tw, int t)
+    {
+        const int w = t >> 6, lane = t & 63;
+        cplx* band = set + w * BAND;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            const int k2 = FW::out_index(lane, i);
+            cplx c = tw[TW_CROSS + w * k2];
+            if (DIR < 0) c.y = -c.y;
+            band[FW::pad(k2)] = cmul(v[i], c);
+        }
+    }
+    FDAS_HD static void cross_finish(cplx* v, const cplx* set, int t)
+    {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int ww = 0; ww < W; ++ww) v[b * W + ww] = set[ww * BAND + FW::pad(t + T * b)];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) dft_reg<W, DIR>(v + b * W);
+    }
+    FDAS_HD static void cross_part(cplx* v, cplx* set, const cplx* tw, int t)
+    {
+        FW::wave_barrier(t & 63);   // the wavefront's last loads from its band are done
+        cross_store(v, set, tw, t);
+        block_sync();
+        cross_finish(v, set, t);
+    }
+    FDAS_HD static void run(cplx* v, cplx* set, const cplx* tw, int t)
+    {
+        wave_part(v, set, tw, t);
+        cross_part(v, set, tw, t);
     }
 };
 

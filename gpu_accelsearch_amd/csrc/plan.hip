@@ -70,18 +70,25 @@ static int store_bank(cplx* bank, int Z, int L, int prec, hipStream_t s, void** 
     return rc;
 }
 
-using BankKey = std::tuple<int, int, int, int, int, int>;   // device, zmax, numharm, uselen, prec
+// device, zmax, numharm, uselen, prec, length mode, row layouts (log2 of each
+// harmonic's wavefront decimation, 4 bits per harmonic)
+using BankKey = std::tuple<int, int, int, int, int, int, long long>;
 static std::mutex g_bank_mu;
 static std::map<BankKey, std::weak_ptr<BankSet>> g_banks;
 
 // Banks of harmonic h have length Ls[h] (a power of two, or R*2^k stored in the
-// decimated order); the lengths follow from (zmax, H, uselen, mode).
+// decimated order); the lengths follow from (zmax, H, uselen, mode).  Rows of a
+// harmonic with decim[h] > 0 are stored in the wavefront-decimated order of the
+// wavefront-split correlation (the split-form rows stay in natural order).
 static int build_banks(int zmax, int H, const int* cur, const int* mx, const int* Ls, int uselen,
-                       int prec, int mode, hipStream_t s, std::shared_ptr<BankSet>* out)
+                       int prec, int mode, const int* decim, hipStream_t s,
+                       std::shared_ptr<BankSet>* out)
 {
     int dev = 0;
     FDAS_HIP(hipGetDevice(&dev));
-    const BankKey key{dev, zmax, H, uselen, prec, mode};
+    long long layout = 0;
+    for (int h = 0; h < H; ++h) layout |= (long long)decim[h] << (4 * h);
+    const BankKey key{dev, zmax, H, uselen, prec, mode, layout};
     std::lock_guard<std::mutex> lk(g_bank_mu);
     auto it = g_banks.find(key);
     if (it != g_banks.end()) {
@@ -116,6 +123,20 @@ static int build_banks(int zmax, int H, const int* cur, const int* mx, const int
             (void)hipFree(bank);
             (void)hipFree(split);
             return rs != FDAS_OK ? rs : FDAS_ERR_HIP;
+        }
+        if (decim[h] > 0) {
+            cplx* dec = nullptr;
+            int rd = hipMalloc(&dec, (size_t)Z * L * sizeof(cplx)) == hipSuccess ? FDAS_OK : FDAS_ERR_NOMEM;
+            if (rd == FDAS_OK) rd = decimate_rows(bank, dec, L, decim[h], Z, s);
+            if (rd == FDAS_OK && hipStreamSynchronize(s) != hipSuccess) rd = FDAS_ERR_HIP;
+            (void)hipFree(bank);
+            bank = dec;
+            if (rd != FDAS_OK) {
+                (void)hipFree(dec);
+                (void)hipFree(split);
+                if (rd == FDAS_ERR_NOMEM) set_error("template bank allocation failed");
+                return rd;
+            }
         }
         const int r1 = store_bank(bank, Z, L, prec, s, &bs->bankq[h], &bs->bank_exp[h], &bs->bytes);
         const int r2 = store_bank(split, Z, L, prec, s, &bs->splitq[h], &bs->split_exp[h], &bs->bytes);
@@ -184,6 +205,8 @@ struct fdas_plan {
     // launch shapes in effect when the plan was made (process-wide choices made
     // later, by the autotuner or the setters, apply to later plans only)
     int corr_var[MAX_HARM]{};
+    int decim[MAX_HARM]{};          // log2 wavefront decimation of segments and rows
+    std::vector<int> class_decim;   // the same per FFT-length class
     int hs_var = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -482,13 +505,22 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     if (chunk <= 0) chunk = (int)std::max<size_t>(1, ((size_t)24 << 30) / per_item);
     pl->chunk = std::min(chunk, pl->nitems);
 
+    // launch shapes in effect now, and the data layouts they read
+    for (int h = 0; h < H; ++h) {
+        pl->corr_var[h] = corr_variant(pl->L[h]);
+        pl->decim[h] = corr_variant_decim(pl->L[h], pl->corr_var[h]);
+    }
+    pl->hs_var = harmsum_variant();
+    pl->class_decim.assign(pl->classes.size(), 0);
+    for (size_t c = 0; c < pl->classes.size(); ++c) pl->class_decim[c] = pl->decim[pl->class_harm[c][0]];
+
     // device state (everything above is host-side validation and geometry)
     if (!pl->stream) {
         FDAS_HIP(hipStreamCreate(&pl->stream));
         pl->own_stream = true;
     }
-    FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, pl->L, U, pl->prec, p.fft_lengths, pl->stream,
-                         &pl->banks));
+    FDAS_TRY(build_banks(p.zmax, H, pl->cur, pl->mx, pl->L, U, pl->prec, p.fft_lengths, pl->decim,
+                         pl->stream, &pl->banks));
     for (size_t c = 0; c < desc.size(); ++c) {
         SegDesc* dd = nullptr;
         FDAS_TRY(plan_alloc(pl, (void**)&dd, desc[c].size() * sizeof(SegDesc)));
@@ -549,8 +581,6 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
     }
     for (int st = 0; st < 5; ++st) hs.powcut[st] = p.powcut[st] > 0.f ? p.powcut[st] : INFINITY;
     hs.cand_cap = pl->cand_cap;
-    for (int h = 0; h < H; ++h) pl->corr_var[h] = corr_variant(pl->L[h]);
-    pl->hs_var = harmsum_variant();
     FDAS_HIP(hipStreamSynchronize(pl->stream));
     return FDAS_OK;
 }
@@ -594,6 +624,7 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
             a.seg_exp = pl->d_seg_exp;
             a.eslot_base = c0 * pl->H;
             a.prec = pl->prec;
+            a.decim = pl->class_decim[c];
             TimedLaunch tl(pl, FDAS_K_SEGPREP);
             FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
         }

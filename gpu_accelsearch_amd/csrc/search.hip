@@ -141,7 +141,7 @@ template <int NH, int R, int P>
 __global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
     const cplx* __restrict__ spec_all, const SegDesc* __restrict__ desc,
     typename CStore<P>::T* __restrict__ seg, long long out_base, int* __restrict__ seg_exp,
-    int eslot_base, const cplx* __restrict__ tw, const cplx* __restrict__ twr)
+    int eslot_base, const cplx* __restrict__ tw, const cplx* __restrict__ twr, int decim)
 {
     constexpr int T = seg_threads<NH, R>();
     using F = BlockFFT<NH, T, -1>;
@@ -193,11 +193,18 @@ __global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
     if constexpr (R > 1) RadixOdd<R, NH, T>::fwd_split(v, t, twr);
     F::template run_multi<R>(v, lds, tw, t);
     typename CStore<P>::T* o = seg + (d.out - out_base);
+    // bin j of the transform at j (natural), or at (j mod W) NH/W + j/W (the
+    // wavefront-decimated order of the wavefront-split correlation, R = 1)
+    auto pos = [&](int r, int i) {
+        const int j = F::out_index(t, i);
+        if (R > 1 || decim == 0) return NH * r + j;
+        return (j & ((1 << decim) - 1)) * (NH >> decim) + (j >> decim);
+    };
     if constexpr (P == PREC_F32) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
 #pragma unroll
-            for (int i = 0; i < E; ++i) o[NH * r + F::out_index(t, i)] = v[r][i];
+            for (int i = 0; i < E; ++i) o[pos(r, i)] = v[r][i];
     } else {
         const int e = block_exponent(block_max_abs<T, R * E>(&v[0][0], red, t));
         if (t == 0) seg_exp[d.eslot - eslot_base] = e;
@@ -205,8 +212,7 @@ __global__ __launch_bounds__((seg_threads<NH, R>())) void seg_prep_kernel(
         for (int r = 0; r < R; ++r)
 #pragma unroll
             for (int i = 0; i < E; ++i)
-                o[NH * r + F::out_index(t, i)] =
-                    CStore<P>::put(mk(ldexpf(v[r][i].x, -e), ldexpf(v[r][i].y, -e)));
+                o[pos(r, i)] = CStore<P>::put(mk(ldexpf(v[r][i].x, -e), ldexpf(v[r][i].y, -e)));
     }
 }
 
@@ -296,6 +302,55 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
         OutE* o = out + ((size_t)s * Z + z) * W;
 #pragma unroll
         for (int i = 0; i < F::E; ++i) {
+            const int col = F::out_index(t, i) - off;
+            if (col >= 0 && col < W)
+                o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
+        }
+    }
+}
+
+// Wavefront-split form of the same correlation at a power of two L (WaveFFT):
+// each of the W = T/64 wavefronts runs a length-L/W inverse transform with
+// exchanges ordered inside the wavefront, then one exchange across wavefronts
+// (one workgroup barrier per step; the steps alternate between two LDS sets).
+// Segment and template rows come in the wavefront-decimated order
+// x'[w L/W + m] = x[w + W m], so the inputs a thread needs, x[w + W (lane + 64 i)],
+// are contiguous over the lanes: K'[w L/W + lane + 64 i], and, the product having
+// period L/2, A'[w L/(2W) + lane + 64 (i mod E/2)].
+template <int L, int G, int T, int P, typename OutE>
+__global__ __launch_bounds__(T) void corr_wave_kernel(
+    const typename CStore<P>::T* __restrict__ bank, const int* __restrict__ bank_exp, int Z,
+    const typename CStore<P>::T* __restrict__ seg, long long seg_stride, long long seg_off,
+    const int* __restrict__ seg_exp, int H, int h, int nsteps, int off, int W,
+    OutE* __restrict__ out, const cplx* __restrict__ tw, float scale, int g0)
+{
+    using F = WaveFFT<L, T, 1>;
+    using S = typename CStore<P>::T;
+    constexpr int E = F::E, EH = E / 2, NW = F::NW;
+    __shared__ cplx lds[2 * F::SET];
+    const int t = threadIdx.x;
+    int z, g;
+    if (!corr_tile(Z, g0, z, g)) return;
+    const int s0 = g * G;
+    const int s1 = G > 1 ? min(s0 + G, nsteps) : s0 + 1;
+    const int wv = t >> 6, lane = t & 63;
+    const S* K = bank + (size_t)z * L + wv * NW + lane;
+    S kr[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) kr[i] = K[64 * i];
+    const int ek = P == PREC_F32 ? 0 : bank_exp[z];
+    for (int s = s0; s < s1; ++s) {
+        const S* A = seg + s * seg_stride + seg_off + wv * (NW / 2) + lane;
+        cplx a[EH], v[E];
+#pragma unroll
+        for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[64 * i]);
+#pragma unroll
+        for (int i = 0; i < E; ++i) v[i] = cmulc(a[i % EH], CStore<P>::get(kr[i]));
+        F::run(v, lds + ((s - s0) & 1) * F::SET, tw, t);
+        const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
+        OutE* o = out + ((size_t)s * Z + z) * W;
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
             const int col = F::out_index(t, i) - off;
             if (col >= 0 && col < W)
                 o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
@@ -481,7 +536,7 @@ static int seg_prep_t(const SegPrepArgs& a, hipStream_t s)
     }
     hipLaunchKernelGGL((seg_prep_kernel<NH, R, P>), dim3(a.nseg), dim3(T), 0, s, a.spec, a.desc,
                        static_cast<typename CStore<P>::T*>(a.seg), a.out_base, a.seg_exp,
-                       a.eslot_base, tw, twr);
+                       a.eslot_base, tw, twr, a.decim);
     return check_launch("seg_prep_kernel");
 }
 
@@ -540,7 +595,8 @@ int launch_segment_prep(int L, const SegPrepArgs& a, hipStream_t s)
 namespace {
 struct CorrShape {
     int T, G, S;   // threads, steps per workgroup; S at powers of two: 1 split form
-                   // (two L/2 transforms), 2 a ring of two LDS regions; at lengths
+                   // (two L/2 transforms), 2 a ring of two LDS regions, 8 the
+                   // wavefront-split form (corr_wave_kernel); at lengths
                    // R*2^k: 0 one region per transform, 1 one region shared by the
                    // R transforms, 2K a ring of two slots of K regions (K
                    // transforms exchanged per barrier)
@@ -558,11 +614,14 @@ struct Shapes {
 FDAS_SHAPES(256, {64, 8, 0})
 FDAS_SHAPES(512, {64, 8, 0})
 FDAS_SHAPES(1024, {64, 8, 0}, {128, 8, 0})
-FDAS_SHAPES(2048, {256, 8, 0}, {128, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2})
-FDAS_SHAPES(4096, {512, 8, 2}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0}, {512, 4, 0},
-            {256, 8, 1}, {128, 8, 1})
-FDAS_SHAPES(8192, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0}, {512, 8, 1},
-            {256, 8, 1}, {512, 8, 2})
+FDAS_SHAPES(2048, {256, 8, 0}, {128, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 2}, {128, 8, 8})
+// The wavefront-split form leads at 4096 and 8192: 10-11% faster than the
+// two-region ring at 4096 (FP32 and I16 storage) and 7% at 8192 (128 templates x
+// 256 steps per launch); slower at 2048.
+FDAS_SHAPES(4096, {512, 16, 8}, {512, 8, 2}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0},
+            {512, 4, 0}, {256, 8, 1}, {128, 8, 1}, {512, 8, 8}, {256, 8, 8})
+FDAS_SHAPES(8192, {512, 16, 8}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0},
+            {512, 8, 1}, {256, 8, 1}, {512, 8, 2}, {512, 8, 8})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
@@ -644,6 +703,14 @@ int set_corr_variant(int L, int v)
     return FDAS_OK;
 }
 
+int corr_variant_decim(int L, int v)
+{
+    int c = 0;
+    const CorrShape* sh = shapes_of(L, &c);
+    if (!sh || v < 0 || v >= c || odd_factor(L) != 1 || sh[v].S != 8) return 0;
+    return ilog2(sh[v].T / 64);
+}
+
 void corr_variant_shape(int L, int v, int* threads, int* steps, int* split, int* form)
 {
     int c = 0;
@@ -677,10 +744,20 @@ static int corr_t(const CorrArgs& a, hipStream_t s)
 {
     using S = typename CStore<P>::T;
     int err = FDAS_OK;
-    const cplx* tw = fft_twiddles_t(SPLIT == 1 ? L / 2 : L, T, &err);
+    const cplx* tw = SPLIT == 8 ? fft_wave_twiddles(L, T, &err)
+                                : fft_twiddles_t(SPLIT == 1 ? L / 2 : L, T, &err);
     if (!tw) return err;
     const float scale = (float)(1.0 / ((double)L * (double)L));
-    if constexpr (SPLIT == 1) {
+    if constexpr (SPLIT == 8) {
+        return corr_launches<T, G>(a, [&](dim3 grid, int g0) {
+            hipLaunchKernelGGL((corr_wave_kernel<L, G, T, P, OutE>), grid, dim3(T), 0, s,
+                               static_cast<const S*>(a.bank), a.bank_exp, a.Z,
+                               static_cast<const S*>(a.seg), a.seg_stride, a.seg_off, a.seg_exp,
+                               a.H, a.h, a.nsteps, a.off, a.W, static_cast<OutE*>(a.out), tw, scale,
+                               g0);
+            return check_launch("corr_wave_kernel");
+        });
+    } else if constexpr (SPLIT == 1) {
         if (!a.bank_split) {
             set_error("split correlation at length %d without a split bank", L);
             return FDAS_ERR_INVALID;

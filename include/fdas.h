@@ -375,7 +375,7 @@ int fdas_rz_power(fdas_plan *plan, int trial, const double *r, const double *z, 
  * each candidate fundamental FFT length (nfftlen 0: 1024..16384), remembering
  * the fastest for plans with fftlen 0 and the same (zmax, numharm, precision). */
 #define FDAS_TUNE_MAX 12
-#define FDAS_TUNE_MAXV 8
+#define FDAS_TUNE_MAXV 16
 typedef struct fdas_tune_report {
     int fftlen;                              /* chosen fundamental length        */
     int ncand;
@@ -393,7 +393,10 @@ typedef struct fdas_tune_report {
     double harmsum_ms[FDAS_TUNE_MAXV];       /* ms per trial per shape (0: n/a)    */
     /* LDS exchange form of corr_variant: 0 one region per transform; 1 the split
      * form (powers of two) or one region shared by the R transforms (lengths
-     * R*2^k); 2K a ring of two slots of K regions, one barrier per exchange */
+     * R*2^k); 2K a ring of two slots of K regions, one barrier per exchange; 8
+     * the wavefront-split form (powers of two: each wavefront transforms 1/W of
+     * the length with exchanges inside the wavefront, then one exchange across
+     * the W wavefronts per step) */
     int corr_form[FDAS_TUNE_MAX];
 } fdas_tune_report;
 

@@ -127,6 +127,68 @@ extern "C" int fft_odd_sim(int r, int m, int t, int dir, const float* in, float*
     return -1;
 }
 
+// Wavefront-split transform (WaveFFT): inputs in the decimated order
+// x[w + W (lane + 64 i)], outputs in natural order; each phase for every thread in
+// turn (the wavefront passes, the cross-wavefront store, then the loads).
+template <int N, int T, int DIR, int P>
+static void run_rest_wave(std::vector<cplx>& v, std::vector<cplx>& set, const std::vector<cplx>& tw)
+{
+    using WF = WaveFFT<N, T, DIR>;
+    using FW = typename WF::FW;
+    constexpr int E = WF::E;
+    if constexpr (P < FW::NPASS) {
+        for (int t = 0; t < T; ++t)
+            FW::template store<P - 1>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
+        for (int t = 0; t < T; ++t)
+            FW::template load<P>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
+        for (int t = 0; t < T; ++t) FW::template compute<P>(&v[(size_t)t * E], t & 63, tw.data());
+        run_rest_wave<N, T, DIR, P + 1>(v, set, tw);
+    }
+}
+
+template <int N, int T, int DIR>
+static void sim_wave(const float* in, float* out)
+{
+    using WF = WaveFFT<N, T, DIR>;
+    using FW = typename WF::FW;
+    constexpr int E = WF::E;
+    std::vector<cplx> tw(WF::TW_COUNT), set(WF::SET);
+    WF::build_twiddles(tw.data());
+    std::vector<cplx> v((size_t)T * E);
+    for (int t = 0; t < T; ++t)
+        for (int i = 0; i < E; ++i) {
+            const int n = WF::in_index(t, i);
+            v[(size_t)t * E + i] = mk(in[2 * n], in[2 * n + 1]);
+        }
+    for (int t = 0; t < T; ++t) FW::template compute<0>(&v[(size_t)t * E], t & 63, tw.data());
+    run_rest_wave<N, T, DIR, 1>(v, set, tw);
+    for (int t = 0; t < T; ++t) WF::cross_store(&v[(size_t)t * E], set.data(), tw.data(), t);
+    for (int t = 0; t < T; ++t) WF::cross_finish(&v[(size_t)t * E], set.data(), t);
+    for (int t = 0; t < T; ++t)
+        for (int i = 0; i < E; ++i) {
+            const int n = WF::out_index(t, i);
+            out[2 * n] = v[(size_t)t * E + i].x;
+            out[2 * n + 1] = v[(size_t)t * E + i].y;
+        }
+}
+
+extern "C" int fft_wave_sim(int n, int t, int dir, const float* in, float* out)
+{
+#define CASE(NN, TT)                                          \
+    if (n == NN && t == TT) {                                 \
+        if (dir < 0) sim_wave<NN, TT, -1>(in, out);           \
+        else sim_wave<NN, TT, +1>(in, out);                   \
+        return 0;                                             \
+    }
+    CASE(2048, 128)
+    CASE(4096, 256)
+    CASE(4096, 512)
+    CASE(8192, 512)
+    CASE(16384, 1024)
+#undef CASE
+    return -1;
+}
+
 // The (N, T) shapes the library instantiates.
 extern "C" int fft_sim(int n, int t, int dir, const float* in, float* out)
 {

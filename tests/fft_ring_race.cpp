@@ -42,6 +42,32 @@ void host_barrier()
     if (g_tid == 0) std::this_thread::sleep_for(std::chrono::microseconds(g_sleep_us));
 }
 
+// A barrier among the 64 threads of the calling thread's wavefront (the lanes of
+// one wavefront execute together on the GPU).
+namespace {
+struct WaveBar {
+    std::mutex mu;
+    std::condition_variable cv;
+    int count = 0;
+    long gen = 0;
+};
+WaveBar g_wave[16];
+}  // namespace
+
+void host_wave_barrier(int)
+{
+    WaveBar& b = g_wave[g_tid >> 6];
+    std::unique_lock<std::mutex> lk(b.mu);
+    const long gen = b.gen;
+    if (++b.count == 64) {
+        b.count = 0;
+        ++b.gen;
+        b.cv.notify_all();
+    } else {
+        b.cv.wait(lk, [&] { return b.gen != gen; });
+    }
+}
+
 }  // namespace fdas
 
 using namespace fdas;
@@ -143,6 +169,63 @@ extern "C" int ring_race(int n, int t, int dir, int na, int nr, int k, int plant
     CASE(1024, 128, 7, 2, 1)
     CASE(1024, 256, 7, 2, 2)
     CASE(4096, 512, 3, 2, 1)
+#undef CASE
+    return -1;
+}
+
+// The wavefront-split transform (WaveFFT) over successive calls, alternating
+// between two LDS sets as the correlation kernel does (one workgroup barrier per
+// call, wavefront barriers inside each wavefront's passes); PLANT reuses one set
+// for every call, which must race.  in/out [calls][N].
+template <int N, int T, int DIR, bool PLANT>
+static void wave_run(int calls, const float* in, float* out)
+{
+    using WF = WaveFFT<N, T, DIR>;
+    constexpr int E = WF::E;
+    std::vector<cplx> tw(WF::TW_COUNT), lds(2 * (size_t)WF::SET);
+    WF::build_twiddles(tw.data());
+    g_total = T;
+    g_count = 0;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            g_tid = t;
+            for (int c = 0; c < calls; ++c) {
+                cplx v[E];
+                for (int i = 0; i < E; ++i) {
+                    const size_t n = (size_t)c * N + WF::in_index(t, i);
+                    v[i] = mk(in[2 * n], in[2 * n + 1]);
+                }
+                WF::run(v, lds.data() + (PLANT ? 0 : (c & 1) * WF::SET), tw.data(), t);
+                for (int i = 0; i < E; ++i) {
+                    const size_t n = (size_t)c * N + WF::out_index(t, i);
+                    out[2 * n] = v[i].x;
+                    out[2 * n + 1] = v[i].y;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
+}
+
+extern "C" int wave_race(int n, int t, int dir, int plant, int calls, int sleep_us, const float* in,
+                         float* out)
+{
+    g_sleep_us = sleep_us;
+#define CASE(NN, TT)                                                                  \
+    if (n == NN && t == TT) {                                                         \
+        if (plant) {                                                                  \
+            if (dir < 0) wave_run<NN, TT, -1, true>(calls, in, out);                  \
+            else wave_run<NN, TT, +1, true>(calls, in, out);                          \
+        } else {                                                                      \
+            if (dir < 0) wave_run<NN, TT, -1, false>(calls, in, out);                 \
+            else wave_run<NN, TT, +1, false>(calls, in, out);                         \
+        }                                                                             \
+        return 0;                                                                     \
+    }
+    CASE(2048, 128)
+    CASE(4096, 256)
+    CASE(4096, 512)
+    CASE(8192, 512)
 #undef CASE
     return -1;
 }

@@ -76,7 +76,7 @@ def test_autotune_selects_and_applies(fd):
         if len(us) > 1:   # the shape in use stays unless another is 3% faster
             assert 0.97 * us[c["variant"]] <= min(us)
         # split form only at powers of two, and reported as LDS form 1 there
-        assert c["lds_form"] in (0, 1, 2, 4)
+        assert c["lds_form"] in (0, 1, 2, 4, 8)
         assert c["split"] == (c["lds_form"] == 1 and l & (l - 1) == 0)
     with fd.AccelSearch(p) as s:
         assert s.info.fftlen[0] == rep["fftlen"]
@@ -124,10 +124,12 @@ def test_plan_keeps_its_launch_shapes(fd):
     try:
         L.fdas_set_corr_variant(8192, 0)
         L.fdas_set_harmsum_variant(0)
+        other_shape = next(v for v, sh in enumerate(fd.corr_variant_shapes(8192))
+                           if sh["threads"] == 1024)   # 1024 threads x 8 values
         with fd.AccelSearch(P) as a:
             a.prepare(x)
             before = a.run_steps(0, 2).copy()
-            L.fdas_set_corr_variant(8192, 4)     # 1024 threads x 8 values
+            L.fdas_set_corr_variant(8192, other_shape)
             L.fdas_set_harmsum_variant(6)
             after = a.run_steps(0, 2)
             np.testing.assert_array_equal(before.view(np.uint32), after.view(np.uint32))

@@ -176,6 +176,20 @@ def test_workgroup_fft_index_arithmetic(fft_sim, n, t):
         assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
 
 
+@pytest.mark.parametrize("n,t", [(2048, 128), (4096, 256), (4096, 512), (8192, 512), (16384, 1024)])
+def test_wavefront_split_transform(fft_sim, n, t):
+    """Length-n transform split across t/64 wavefronts (WaveFFT): inputs in the
+    decimated order x[w + W (lane + 64 i)], outputs in natural order."""
+    rng = np.random.default_rng(n + 3 * t)
+    for sign in (-1, 1):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+        out = np.zeros(n, np.complex64)
+        rc = fft_sim.fft_wave_sim(n, t, sign, C.c_void_p(x.ctypes.data), C.c_void_p(out.ctypes.data))
+        assert rc == 0
+        ref = np.fft.fft(x.astype(np.complex128)) if sign < 0 else np.fft.ifft(x.astype(np.complex128)) * n
+        assert np.max(np.abs(out - ref)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 1e-6
+
+
 @pytest.mark.parametrize("r,m,t", [(3, 128, 64), (3, 256, 64), (3, 512, 64), (3, 1024, 64),
                                    (3, 1024, 128), (3, 2048, 128), (3, 2048, 256), (5, 128, 64),
                                    (5, 256, 64), (5, 1024, 128), (5, 1024, 256), (7, 256, 64),
@@ -235,8 +249,9 @@ def test_corr_variant_shapes_table():
     """Every correlation length's launch shapes (host tables, no GPU): at most
     FDAS_TUNE_MAXV per length, threads a power of two dividing the transform
     length with at least 2 values each, the split form only at powers of two,
-    and an LDS ring of 2K regions only for K <= R transforms; out-of-range
-    variants are rejected."""
+    the wavefront-split form only at powers of two with its radix-W step
+    dividing the values per thread (W = threads/64 >= 2), and an LDS ring of 2K
+    regions only for K <= R transforms; out-of-range variants are rejected."""
     import ctypes as C
     import gpu_accelsearch_amd as fd
     from gpu_accelsearch_amd import _native as N
@@ -251,8 +266,11 @@ def test_corr_variant_shapes_table():
             t, form = sh["threads"], sh["lds_form"]
             assert t & (t - 1) == 0 and m % t == 0 and m // t >= 2, (L, sh)
             if r == 1:
-                assert form in (0, 1, 2), (L, sh)
+                assert form in (0, 1, 2, 8), (L, sh)
                 assert sh["split"] == (form == 1)
+                if form == 8:
+                    w = t // 64
+                    assert w >= 2 and (m // t) % w == 0, (L, sh)
             else:
                 assert not sh["split"] and (form in (0, 1) or (form % 2 == 0 and form // 2 <= r)), (L, sh)
         z = C.c_int()

@@ -84,3 +84,35 @@ def test_other_exchange_forms_have_no_race(ring, n, t, na, form):
         ref = np.fft.fft(xd, axis=-1) if sign < 0 else np.fft.ifft(xd, axis=-1) * n
         err = np.abs(out - ref).max() / np.sqrt(np.mean(np.abs(ref) ** 2))
         assert err < 1e-6, (sign, err)
+
+
+@pytest.mark.parametrize("n,t", [(2048, 128), (4096, 256), (4096, 512), (8192, 512)])
+def test_wavefront_split_has_no_race(ring, n, t):
+    """WaveFFT (wavefront-local passes, one exchange across wavefronts) over
+    successive calls that alternate between two LDS sets, as the correlation
+    kernel's steps do."""
+    ring.wave_race.argtypes = [C.c_int] * 6 + [C.c_void_p, C.c_void_p]
+    calls = 4
+    rng = np.random.default_rng(n + t + 11)
+    for sign in (1, -1):
+        x = (rng.standard_normal((calls, n)) + 1j * rng.standard_normal((calls, n))).astype(np.complex64)
+        out = np.zeros_like(x)
+        assert ring.wave_race(n, t, sign, 0, calls, SLEEP_US, x.ctypes.data, out.ctypes.data) == 0
+        xd = x.astype(np.complex128)
+        ref = np.fft.fft(xd, axis=-1) if sign < 0 else np.fft.ifft(xd, axis=-1) * n
+        err = np.abs(out - ref).max() / np.sqrt(np.mean(np.abs(ref) ** 2))
+        assert err < 1e-6, (sign, err)
+
+
+def test_wavefront_split_check_sees_one_set_reused(ring):
+    """A planted mistake: one LDS set for every call lets a wavefront start the
+    next call's passes in its band while another still reads it."""
+    ring.wave_race.argtypes = [C.c_int] * 6 + [C.c_void_p, C.c_void_p]
+    n, t, calls = 4096, 256, 3
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal((calls, n)) + 1j * rng.standard_normal((calls, n))).astype(np.complex64)
+    out = np.zeros_like(x)
+    assert ring.wave_race(n, t, 1, 1, calls, SLEEP_US, x.ctypes.data, out.ctypes.data) == 0
+    ref = np.fft.ifft(x.astype(np.complex128), axis=-1) * n
+    err = np.abs(out - ref).max() / np.sqrt(np.mean(np.abs(ref) ** 2))
+    assert err > 1e-2, err
