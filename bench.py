@@ -72,11 +72,16 @@ def corr_flops(info) -> dict:
             "corr_fund": S * f(0) + 3 * info.cells}
 
 
-def measured_traffic(kernel: str):
+def measured_traffic(kernel: str, precision: str = "fp32"):
     """HBM bytes per launch of a kernel category from the committed counter
-    summary of the FP32 production trial (rocprofv3 --pmc FETCH_SIZE, then
-    WRITE_SIZE, scaled by the same counters on a copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", "production_trial_traffic.json")
+    summary of the production trial in this storage mode (rocprofv3 --pmc
+    FETCH_SIZE, then WRITE_SIZE, scaled by the same counters on a copy of known
+    size), or None where no summary was taken."""
+    name = {"fp32": "production_trial_traffic.json",
+            "i16": "production_trial_i16_traffic.json"}.get(precision)
+    if name is None:
+        return None
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             d = json.load(f)
@@ -280,10 +285,26 @@ def serving_form(count, rate, sweep, sweep_count, seed=6):
             runs.append({key: s[key] for key in ("offered_rate_per_s", "requests", "wall_s",
                                                  "dm_trials_per_s", "p50_ms", "p99_ms",
                                                  "max_active_seen", "failed")})
+        # the same mix one request at a time (submit, wait): the device time a
+        # server without overlap would need, against which the capacity is judged
+        srv.reset_stats()
+        reqs = synth.serving_stream(sweep_count, 1.0, seed=seed + 10)
+        t0 = time.perf_counter()
+        for r in reqs:
+            x = pool[r["offset"]:r["offset"] + r["n"]]
+            if r["pulsar"]:
+                x = x + synth.pulse_train(r["n"], 0.37 * r["n"] / 2, 0.5 * r["zmax"], 0.05,
+                                          30.0).astype(np.float32)
+            p = fd.SearchParams(n=r["n"], zmax=r["zmax"], numharm=r["numharm"], fftlen=0,
+                                precision=r["precision"], powcut=(40.0, 45.0, 55.0, 70.0, 90.0))
+            srv.wait(srv.submit(p, x))
+        serial = time.perf_counter() - t0
         budget = srv.stats()["memory_bytes"]
     main["memory_budget_bytes"] = budget
     main["rate_sweep"] = runs
     main["capacity_dm_trials_per_s"] = max(r["dm_trials_per_s"] for r in runs + [main])
+    main["serial_dm_trials_per_s"] = sweep_count / serial
+    main["overlap_gain"] = main["capacity_dm_trials_per_s"] / main["serial_dm_trials_per_s"]
     main["mix"] = ("n log-uniform [2^20, 2^25], zmax {50,200,600}, numharm {4,8,16}, precision "
                    "{fp32, i16}, 5% with a pulsar; fftlen per request from the autotuned/default "
                    "table; max_active 4, 128 steps per launch group")
@@ -316,6 +337,9 @@ def main() -> int:
     ap.add_argument("--serve-sweep", default="20,100,200",
                     help="further offered rates (req/s) for the capacity sweep")
     ap.add_argument("--serve-sweep-requests", type=int, default=150)
+    ap.add_argument("--lengths", default="shortest", choices=["shortest", "reference"],
+                    help="sub-harmonic FFT lengths: the shortest that fit, or the "
+                         "reference's powers of two")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default correlation shapes (no autotuner launches)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -334,7 +358,8 @@ def main() -> int:
     n, zmax, numharm, fftlen = 1 << 23, 200, 8, 8192
     series = synth.production_series(n)
     params = SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen,
-                          chunk_steps=args.chunk, precision=args.precision, powcut=POWCUT)
+                          chunk_steps=args.chunk, precision=args.precision, powcut=POWCUT,
+                          fft_lengths=args.lengths)
     # correlation launch shapes measured for this configuration's FFT lengths
     tune = None if args.no_tune else fd.autotune(params, fftlens=(fftlen,))
     s = AccelSearch(params)
@@ -415,13 +440,13 @@ def main() -> int:
         "config": {"workload": "production DM trial: 2^23 samples, dt 64 us, zmax 200 "
                                "(201 templates), fftlen 8192 (uselen 7734), numharm 8, "
                                f"{info.nsteps} steps, {info.cells} cells",
-                   "chunk_steps": info.chunk_steps},
+                   "chunk_steps": info.chunk_steps, "fft_lengths": args.lengths},
         "dm_trials_per_s": 1.0 / per_trial,
         "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
                        "p99": float(np.percentile(lat, 99) * 1e3)},
         "roofline": {"bound": "hbm", "kernel": "harmsum_kernel", "achieved": rf["achieved_GBps"],
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": rf["frac"],
-                     "traffic": measured_traffic("harmsum") if args.precision == "fp32" else None,
+                     "traffic": measured_traffic("harmsum", args.precision),
                      "alg_bytes_per_launch": hs_bytes, "avg_launch_ms": hs_ms},
         "kernel_rooflines": kernel_rooflines,
         "trial_roofline": {"alg_bytes": ab["trial"], "frac": ab["trial"] / per_trial / HBM_PEAK},
