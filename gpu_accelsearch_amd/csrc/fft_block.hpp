@@ -122,15 +122,47 @@ FDAS_HD cplx twc(cplx b)
     }
 }
 
+// Radix-2 butterfly (a, b) -> (a + w b, a - w b), w = exp(DIR 2 pi i M / 64) for a
+// compile-time M, with the twiddle folded into fused multiply-adds: at M = 8 and
+// 24 (w = (+-1 +- i)/sqrt 2) the sums b.x +- b.y are scaled inside the fused
+// operations, and at a general M, a + w b takes four and a - w b = 2a - (a + w b)
+// two, so a twiddled butterfly issues 6 instructions instead of 8.
+template <int DIR, int M>
+FDAS_HD void bfly(cplx& a, cplx& b)
+{
+    if constexpr (M == 0 || M == 16) {
+        const cplx wb = twc<DIR, M>(b), t = a;
+        a = t + wb;
+        b = t - wb;
+    } else if constexpr (M == 8 || M == 24) {
+        constexpr float h = 7.071067691e-01f;
+        // w b = h (px, py)
+        float px, py;
+        if constexpr (M == 8) {
+            px = DIR > 0 ? b.x - b.y : b.x + b.y;
+            py = DIR > 0 ? b.x + b.y : b.y - b.x;
+        } else {
+            px = DIR > 0 ? -(b.x + b.y) : b.y - b.x;
+            py = DIR > 0 ? b.x - b.y : -(b.x + b.y);
+        }
+        const cplx t = a;
+        a = mk(fmaf(px, h, t.x), fmaf(py, h, t.y));
+        b = mk(fmaf(-px, h, t.x), fmaf(-py, h, t.y));
+    } else {
+        constexpr float c = kCos64[M], s = DIR > 0 ? kSin64[M] : -kSin64[M];
+        const float y0x = fmaf(b.x, c, fmaf(-b.y, s, a.x));
+        const float y0y = fmaf(b.x, s, fmaf(b.y, c, a.y));
+        b = mk(fmaf(2.f, a.x, -y0x), fmaf(2.f, a.y, -y0y));
+        a = mk(y0x, y0y);
+    }
+}
+
 // In-register radix-2 stages (all indices compile-time, unrolled by recursion).
 template <int DIR, int LEN, int I, int K>
 FDAS_HD void dft_butterflies(cplx* v)
 {
     if constexpr (K < LEN / 2) {
-        cplx a = v[I + K];
-        cplx b = twc<DIR, K*(64 / LEN)>(v[I + K + LEN / 2]);
-        v[I + K] = a + b;
-        v[I + K + LEN / 2] = a - b;
+        bfly<DIR, K*(64 / LEN)>(v[I + K], v[I + K + LEN / 2]);
         dft_butterflies<DIR, LEN, I, K + 1>(v);
     }
 }

@@ -8,13 +8,14 @@
 //
 // One workgroup owns a tile of HS_RJ rows x HS_CB columns of one step (4 x 896 or
 // 4 x 768 by default).  The cells of sub-harmonic h that the tile reads form a
-// rectangle of about (HS_RJ f + 1) x
-// (HS_CB f + 1) values; the rectangles of as many harmonics as fit are staged in
-// LDS with 16-byte loads and stores (rows start on 4-float boundaries: sub-plane
-// rows are padded to multiples of 4), all loads of a round issued before any
-// store, and the sum gathers from LDS.  Each fundamental cell thus costs about
-// sum_h f_h^2 distinct sub-harmonic reads (2.19 at 8 harmonics) instead of H-1
-// scattered ones.
+// rectangle of about HS_CB f + 1 columns of the sub-harmonic rows its fundamental
+// rows map to.  The rectangles of as many harmonics as fit are staged in LDS
+// column by column: a column's 16-byte slot holds, in fundamental-row order, the
+// value each of the tile's rows reads there (the row map is uniform over the
+// workgroup, so it is applied once, in the staging loads), and one 16-byte LDS
+// read per column then gives a thread the addends of all its rows with no per-cell
+// address arithmetic.  Each fundamental cell thus costs about sum_h f_h^2 distinct
+// sub-harmonic reads (2.19 at 8 harmonics) instead of H-1 scattered ones.
 #include <atomic>
 #include <cstdlib>
 
@@ -24,36 +25,20 @@
 
 namespace fdas {
 
-// Tile shapes: HS_T threads, HS_RJ rows x HS_CB columns, HS_LDS staged floats.
-// HS_MAXR sub rows and HS_MAXK loads per sub row and thread bound one rectangle.
+// Tile shapes: HS_T threads, HS_RJ (2..4) rows x HS_CB columns, HS_LDS staged
+// floats.  A rectangle has at most HS_CB + 2 columns of 4 floats (one per tile
+// row, the unused ones zero); HS_MAXK column slots per thread cover it.
 #define HS_TILE_CONSTANTS                                                      \
     constexpr int HS_CPT = HS_CB / HS_T;                                       \
-    constexpr int HS_MAXR = HS_RJ + 1;                                         \
-    constexpr int HS_MAXQ = ((HS_CB + 8) / 4 + HS_T - 1) / HS_T;               \
-    static_assert(HS_MAXR * (HS_CB + 8) <= HS_LDS, "one rectangle must fit");  \
+    constexpr int HS_MAXK = (HS_CB + 2 + HS_T - 1) / HS_T;                     \
+    static_assert(HS_RJ >= 2 && HS_RJ <= 4, "rows of a tile");                 \
+    static_assert(4 * (HS_CB + 2) <= HS_LDS, "one rectangle must fit");        \
     static_assert(HS_CB % HS_T == 0 && HS_RJ * HS_CPT <= 32, "a hit bit per cell");
 
-// Four consecutive plane elements as one load (16 bytes of FP32, 8 of P16), and
-// their FP32 values.
-template <typename PE>
-struct Quad;
-template <>
-struct Quad<float> {
-    using V = float4;
-    __device__ __forceinline__ static V zero() { return make_float4(0.f, 0.f, 0.f, 0.f); }
-    __device__ __forceinline__ static float4 values(V v) { return v; }
-};
-template <>
-struct Quad<unsigned short> {
-    using V = uint2;
-    __device__ __forceinline__ static V zero() { return make_uint2(0u, 0u); }
-    __device__ __forceinline__ static float4 values(V v)
-    {
-        using P = PStore<unsigned short>;
-        return make_float4(P::get((unsigned short)(v.x & 0xffffu)), P::get((unsigned short)(v.x >> 16)),
-                           P::get((unsigned short)(v.y & 0xffffu)), P::get((unsigned short)(v.y >> 16)));
-    }
-};
+__device__ __forceinline__ float comp4(const float4& v, int i)
+{
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+}
 
 // grid: (ceil(U/HS_CB), ceil(Z0/HS_RJ), steps); fund[s][Z0][U] holds the
 // fundamental powers (plane element PE: FP32 or P16) and, when store is set, the
@@ -127,11 +112,11 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
 
     // Geometry of every harmonic's rectangle, computed once per tile by one
     // thread per harmonic and read back by all waves as uniform values:
-    // [0] first staged column (4-aligned), [1] staged row width (multiple of 4),
-    // [2] rows, [3] c, [4] log2 m, [5] 2 rlo_h, [6 + r] row of fundamental row r
-    // relative to the first, [6 + HS_RJ] LDS offset of the rectangle in its
-    // round; gbase[h]: element offset of the rectangle's first staged value.
-    constexpr int GEO = 7 + HS_RJ;
+    // [0] first staged column, [1] staged columns, [2] c, [3] log2 m, [4] 2 rlo_h,
+    // [5] LDS offset of the rectangle in its round, [6 + r] sub-harmonic row read
+    // by fundamental row r relative to row 0's; gbase[h]: element offset of the
+    // rectangle's first value (row 0's sub-harmonic row, column [0]).
+    constexpr int GEO = 6 + HS_RJ;
     __shared__ int geo[MAX_HARM][GEO];
     __shared__ long long gbase[MAX_HARM];
     __shared__ int rounds[MAX_HARM + 1];
@@ -139,17 +124,16 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
     if (t > 0 && t < hs.H) {
         const int c = hs.c[t], lm = hs.lm[t], r2 = 2 * rl[t];
         const int zi0 = hs_zind(hs.zlo0 + DZ * j0, c, lm, hs.zlo[t]);
-        const int ri0a = hs_rind(q0 + c0, c, lm, r2) & ~3;
-        geo[t][0] = ri0a;
-        geo[t][1] = (hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0a + 3) & ~3;
-        geo[t][2] = hs_zind(hs.zlo0 + DZ * (j0 + nrow - 1), c, lm, hs.zlo[t]) - zi0 + 1;
-        geo[t][3] = c;
-        geo[t][4] = lm;
-        geo[t][5] = r2;
+        const int ri0 = hs_rind(q0 + c0, c, lm, r2);
+        geo[t][0] = ri0;
+        geo[t][1] = hs_rind(q0 + c0 + ncol - 1, c, lm, r2) + 1 - ri0;
+        geo[t][2] = c;
+        geo[t][3] = lm;
+        geo[t][4] = r2;
 #pragma unroll
         for (int r = 0; r < HS_RJ; ++r)
             geo[t][6 + r] = hs_zind(hs.zlo0 + DZ * (j0 + min(r, nrow - 1)), c, lm, hs.zlo[t]) - zi0;
-        gbase[t] = hs.sub_off[t] + (long long)s * hs.sub_stride[t] + (long long)zi0 * hs.W[t] + ri0a;
+        gbase[t] = hs.sub_off[t] + (long long)s * hs.sub_stride[t] + (long long)zi0 * hs.W[t] + ri0;
     }
     __syncthreads();
     if (t == 0) {
@@ -157,12 +141,12 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
         int n = 0, used = 0;
         rounds[0] = 1;
         for (int h = 1; h < hs.H; ++h) {
-            const int sz = geo[h][1] * geo[h][2];
+            const int sz = 4 * geo[h][1];
             if (used + sz > HS_LDS) {
                 rounds[++n] = h;
                 used = 0;
             }
-            geo[h][6 + HS_RJ] = used;
+            geo[h][5] = used;
             used += sz;
         }
         rounds[++n] = hs.H;
@@ -177,53 +161,64 @@ __global__ __launch_bounds__(HS_T) void harmsum_kernel(const PE* fund, float* ou
         const int h = u(rounds[rd]), h2 = u(rounds[rd + 1]);
         if (rd > 0) __syncthreads();   // earlier readers of the tile are done
         for (int g = h; g < h2; ++g) {
-            const int nca = u(geo[g][1]), nr = u(geo[g][2]), base = u(geo[g][6 + HS_RJ]);
-            const int nq = nca >> 2;
-            using Q = Quad<PE>;
-            const typename Q::V* src = reinterpret_cast<const typename Q::V*>(sub + gbase[g]);
-            const int wq = hs.W[g] >> 2;
-            typename Q::V v[HS_MAXR][HS_MAXQ];
+            const int nca = u(geo[g][1]), base = u(geo[g][5]);
+            // column slots t + k HS_T: the value each tile row reads in that column
+            // (loaded once per distinct sub-harmonic row, clamped to the last
+            // column so every address is inside the rectangle), all loads before
+            // any store, then one 16-byte LDS store per column
+            // uniform row pointers (so the loads take a scalar base and a 32-bit
+            // per-lane offset)
+            const long long gb = gbase[g];
+            const long long gbu = ((long long)u((int)(gb >> 32)) << 32) | (unsigned)u((int)gb);
+            const int W = hs.W[g];
+            const PE* rowp[HS_RJ];
 #pragma unroll
-            for (int rr = 0; rr < HS_MAXR; ++rr)
+            for (int r = 0; r < HS_RJ; ++r) rowp[r] = sub + gbu + (long long)(u(geo[g][6 + r]) * W);
+            // every row loaded (a row that reads the same sub-harmonic row as the
+            // row before it reads the same cache lines), all loads before any store
+            PE e[HS_MAXK][HS_RJ];
 #pragma unroll
-                for (int k = 0; k < HS_MAXQ; ++k) {
-                    const int q = t + k * HS_T;
-                    v[rr][k] = (rr < nr && q < nq) ? src[(long long)rr * wq + q] : Q::zero();
+            for (int k = 0; k < HS_MAXK; ++k) {
+                if (k * HS_T < nca) {
+                    const unsigned off = (unsigned)min(t + k * HS_T, nca - 1) * (unsigned)sizeof(PE);
+#pragma unroll
+                    for (int r = 0; r < HS_RJ; ++r)
+                        e[k][r] = *reinterpret_cast<const PE*>(reinterpret_cast<const char*>(rowp[r]) + off);
                 }
+            }
             if (first) {
                 mark_fundamental();
                 first = false;
             }
             float4* dst = reinterpret_cast<float4*>(tile + base);
 #pragma unroll
-            for (int rr = 0; rr < HS_MAXR; ++rr)
+            for (int k = 0; k < HS_MAXK; ++k) {
+                const int cc = t + k * HS_T;
+                if (k * HS_T < nca && cc < nca) {
+                    float f[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int k = 0; k < HS_MAXQ; ++k) {
-                    const int q = t + k * HS_T;
-                    if (rr < nr && q < nq) dst[rr * nq + q] = Q::values(v[rr][k]);
+                    for (int r = 0; r < HS_RJ; ++r) f[r] = PStore<PE>::get(e[k][r]);
+                    dst[cc] = make_float4(f[0], f[1], f[2], f[3]);
                 }
+            }
         }
         __syncthreads();
         for (int g = h; g < h2; ++g) {
-            const int ri0a = u(geo[g][0]), nca = u(geo[g][1]), base = u(geo[g][6 + HS_RJ]);
-            const int c = u(geo[g][3]), lm = u(geo[g][4]), r2 = u(geo[g][5]);
+            const int ri0 = u(geo[g][0]), base = u(geo[g][5]);
+            const int c = u(geo[g][2]), lm = u(geo[g][3]), r2 = u(geo[g][4]);
             // staged column of each of the thread's columns: hs_rind with the
             // product q c stepped by HS_T c from column to column, clamped to the
             // tile's last column (cells beyond it are -inf and never used)
-            const int ri_last = u(hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0a);
+            const int ri_last = u(hs_rind(q0 + c0 + ncol - 1, c, lm, r2) - ri0);
             unsigned qc = (q0 + (unsigned)(c0 + t)) * (unsigned)c + (1u << (lm - 1));
             const unsigned qstep = (unsigned)(HS_T * c);
-            int ri[HS_CPT];
+            const float4* col = reinterpret_cast<const float4*>(tile + base);
 #pragma unroll
             for (int k = 0; k < HS_CPT; ++k) {
-                ri[k] = base + min((int)(qc >> lm) - (r2 + ri0a), ri_last);
+                const float4 cv = col[min((int)(qc >> lm) - (r2 + ri0), ri_last)];
                 qc += qstep;
-            }
 #pragma unroll
-            for (int r = 0; r < HS_RJ; ++r) {
-                const int zi = u(geo[g][6 + r]);
-#pragma unroll
-                for (int k = 0; k < HS_CPT; ++k) acc[r][k] += tile[zi * nca + ri[k]];
+                for (int r = 0; r < HS_RJ; ++r) acc[r][k] += comp4(cv, r);
             }
             if (((g + 1) & g) == 0) mark_cells(hs.powcut[++stage]);
         }
@@ -316,8 +311,8 @@ struct HsShape {
     int T, RJ, CB, LDS;   // CB 0: 896 or 768 chosen per launch from U
 };
 constexpr HsShape kHsShapes[] = {{128, 4, 0, 4608}, {128, 4, 768, 4096}, {128, 4, 896, 4608},
-                                 {128, 3, 1024, 4160}, {128, 2, 1024, 4096}, {128, 3, 896, 3840},
-                                 {256, 2, 2048, 8192}, {128, 5, 640, 4096}};
+                                 {128, 3, 1024, 4160}, {128, 2, 1024, 4160}, {128, 3, 896, 3840},
+                                 {256, 2, 2048, 8256}, {128, 4, 640, 3072}};
 constexpr int kHsCount = (int)(sizeof(kHsShapes) / sizeof(kHsShapes[0]));
 std::atomic<int> g_hs_variant{0};
 
@@ -356,10 +351,10 @@ int launch_harmsum(const void* d_fund, float* d_out, const void* d_sub, int prec
         case 1: return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
         case 2: return harmsum_t<128, 4, 896, 4608>(HS_ARGS);
         case 3: return harmsum_t<128, 3, 1024, 4160>(HS_ARGS);
-        case 4: return harmsum_t<128, 2, 1024, 4096>(HS_ARGS);
+        case 4: return harmsum_t<128, 2, 1024, 4160>(HS_ARGS);
         case 5: return harmsum_t<128, 3, 896, 3840>(HS_ARGS);
-        case 6: return harmsum_t<256, 2, 2048, 8192>(HS_ARGS);
-        case 7: return harmsum_t<128, 5, 640, 4096>(HS_ARGS);
+        case 6: return harmsum_t<256, 2, 2048, 8256>(HS_ARGS);
+        case 7: return harmsum_t<128, 4, 640, 3072>(HS_ARGS);
         default:
             if (tail_waste(hs.U, 896) <= tail_waste(hs.U, 768)) return harmsum_t<128, 4, 896, 4608>(HS_ARGS);
             return harmsum_t<128, 4, 768, 4096>(HS_ARGS);
