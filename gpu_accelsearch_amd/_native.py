@@ -16,7 +16,7 @@ MAX_STAGES = 5
 FDAS_OK = 0
 ERRORS = {1: "invalid argument", 2: "HIP runtime error", 3: "out of device memory",
           4: "unsupported shape", 5: "candidate capacity exceeded"}
-PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "i16": 3}
+PREC = {"fp32": 0, "fp16": 1, "bf16": 2, "i16": 3, "i16mma": 4}
 LENGTHS = {"shortest": 0, "reference": 1}
 
 
@@ -214,6 +214,12 @@ PROTOTYPES = [
     ("fdas_set_harmsum_variant", i32, [i32]),
     ("fdas_get_corr_variant", i32, [i32]),
     ("fdas_set_corr_variant", i32, [i32, i32]),
+    ("fdas_mma_variant_count", i32, []),
+    ("fdas_get_mma_variant", i32, []),
+    ("fdas_set_mma_variant", i32, [i32]),
+    ("fdas_mma_variant_units", i32, [i32]),
+    ("fdas_mma_variant_steps", i32, [i32]),
+    ("fdas_mma_variant_threads", i32, [i32]),
     ("fdas_corr_variant_shape", i32, [i32, i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
     ("fdas_resolve_fftlen", i32, [C.POINTER(FdasParams)]),
     ("fdas_harmonic_fftlens", i32, [C.POINTER(FdasParams), C.POINTER(C.c_int), C.POINTER(C.c_int)]),

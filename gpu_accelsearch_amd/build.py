@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfdas.so")
-SOURCES = ["api.hip", "plan.hip", "serve.hip", "tune.hip", "sift.hip", "io.hip", "fft.hip", "whiten.hip", "templates.hip", "search.hip",
+SOURCES = ["api.hip", "plan.hip", "serve.hip", "tune.hip", "sift.hip", "io.hip", "fft.hip", "whiten.hip", "templates.hip", "search.hip", "direct.hip",
            "harmsum.hip"]
 ARCH = os.environ.get("FDAS_OFFLOAD_ARCH", "gfx950")
 

@@ -152,6 +152,8 @@ struct SegDesc {          // one (step, harmonic) segment of the overlap-save lo
     int eslot;            // item * H + harmonic: slot of its storage exponent
     int nbins;            // bins of its trial's spectrum (reads beyond give zero)
     long long spec_off;   // offset of its trial's spectrum in the spectrum buffer
+    int nch;              // direct correlation: 16-byte chunks per byte array
+                          // (out is then in chunks: hi array, lo array after it)
 };
 
 struct HarmSumArgs {
@@ -222,6 +224,46 @@ int launch_bank_split(const cplx* d_bank, int Z, int L, cplx* d_out, hipStream_t
 // Reduced-precision copy of a bank of Z rows of length L.
 int launch_bank_pack(const cplx* d_bank, int Z, int L, int prec, void* d_out, int* d_row_exp,
                      hipStream_t s);
+
+// Direct correlation on the matrix cores (precision FDAS_PREC_I16_MMA, direct.hip;
+// index arithmetic in mma_layout.hpp).  A tile is two template rows; its B
+// fragments (nk K steps x 4 x 64 lanes x 16 bytes) start at boff (16-byte units).
+struct MmaTile {
+    int row[2];        // plane rows of its templates (row[1] = -1: none)
+    int jlo, nk;       // first tap block, K steps
+    long long boff;
+};
+struct MmaBank {
+    void* frags = nullptr;
+    MmaTile* tiles = nullptr;
+    float* row_inv = nullptr;   // per row: 1 / quantisation scale
+    int ntiles = 0, nk_max = 0;
+    size_t bytes = 0;
+};
+// Bank of the Z rows z = -zmax_h + 2k of a harmonic whose largest half-width is bo.
+int build_mma_bank(int zmax_h, int Z, int bo, hipStream_t s, MmaBank* out);
+void free_mma_bank(MmaBank* b);
+// Segment preparation for the direct correlation (segments of FFT-length class L,
+// L/2 values); a.seg_exp receives each segment's inverse scale as a float.
+int launch_segment_prep_mma(int L, const SegPrepArgs& a, hipStream_t s);
+struct MmaCorrArgs {
+    const MmaBank* bank;
+    const void* seg;           // chunk buffer (16-byte units)
+    long long seg_stride;      // chunks per step
+    long long seg_off;         // first chunk of this harmonic's hi array
+    int nch;                   // chunks per byte array
+    const float* seg_inv;      // [step * H + h]
+    int H, h, nsteps, W, Z, nm;
+    void* out;                 // P16 plane [step][Z][W]
+    int variant;               // launch shape (mma_variant at plan creation)
+};
+int launch_corr_mma(const MmaCorrArgs& a, hipStream_t s);
+int mma_variant_count();
+int mma_variant();
+int set_mma_variant(int v);
+int mma_variant_units(int v);   // M tiles per wavefront of a variant
+int mma_variant_steps(int v);   // steps per workgroup of a variant
+int mma_variant_threads(int v); // threads per workgroup of a variant
 
 // Harmonic sum over d_fund[step][Z0][U] (fundamental powers) and the sub-harmonic
 // planes d_sub, both in the plane element of storage mode prec (FP32, or P16 in the

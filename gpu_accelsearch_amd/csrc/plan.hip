@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "fdas_internal.hpp"
+#include "mma_layout.hpp"
 
 using namespace fdas;
 
@@ -31,10 +32,12 @@ struct BankSet {
     int* bank_exp[MAX_HARM]{};    // per-row exponents (reduced precision)
     void* splitq[MAX_HARM]{};     // split-form rows [E | O] for the split correlation
     int* split_exp[MAX_HARM]{};
+    MmaBank mma[MAX_HARM]{};      // direct-correlation banks (FDAS_PREC_I16_MMA)
     size_t bytes = 0;
     ~BankSet()
     {
         for (int h = 0; h < MAX_HARM; ++h) {
+            free_mma_bank(&mma[h]);
             (void)hipFree(bankq[h]);
             (void)hipFree(bank_exp[h]);
             (void)hipFree(splitq[h]);
@@ -105,6 +108,11 @@ static int build_banks(int zmax, int H, const int* cur, const int* mx, const int
         const int Z = (zmax_h / DZ) * 2 + 1, L = Ls[h];
         bs->Z[h] = Z;
         bs->L[h] = L;
+        if (prec == FDAS_PREC_I16_MMA) {   // time-domain tiles, no FFT'd rows
+            FDAS_TRY(build_mma_bank(zmax_h, Z, h_z_response_halfwidth((double)zmax_h), s, &bs->mma[h]));
+            bs->bytes += bs->mma[h].bytes;
+            continue;
+        }
         cplx *bank = nullptr, *split = nullptr;
         FDAS_HIP(hipMalloc(&bank, (size_t)Z * L * sizeof(cplx)));
         FDAS_TRY(build_template_bank(bank, zmax_h, Z, L, s));
@@ -179,7 +187,10 @@ struct fdas_plan {
         off[MAX_HARM]{}, W[MAX_HARM]{}, zlo[MAX_HARM]{};
     std::shared_ptr<BankSet> banks;
     int prec = FDAS_PREC_FP32;
-    size_t cbytes = sizeof(cplx);
+    bool mma = false;               // direct correlation (FDAS_PREC_I16_MMA)
+    int nm[MAX_HARM]{}, nch[MAX_HARM]{};   // its M tiles and staged chunks per harmonic
+    int mma_var = 0;                // its launch shape
+    size_t cbytes = sizeof(cplx);   // per segment element (mma: a 16-byte chunk)
     long long seg_off[MAX_HARM]{};
     long long seg_stride = 0;
     long long sub_total = 0;
@@ -277,7 +288,7 @@ int validate(const fdas_params& p)
         set_error("fftlen %d must be a power of two in [1024, 16384]", p.fftlen);
         return FDAS_ERR_UNSUPPORTED;
     }
-    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_I16) {
+    if (p.precision < FDAS_PREC_FP32 || p.precision > FDAS_PREC_I16_MMA) {
         set_error("precision mode %d unknown", p.precision);
         return FDAS_ERR_INVALID;
     }
@@ -296,7 +307,8 @@ int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
     const fdas_params& p = pl->p;
     FDAS_TRY(validate(p));
     pl->prec = p.precision;
-    pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : 4;
+    pl->mma = p.precision == FDAS_PREC_I16_MMA;
+    pl->cbytes = p.precision == FDAS_PREC_FP32 ? sizeof(cplx) : pl->mma ? 16 : 4;
     pl->pbytes = p.precision == FDAS_PREC_FP32 ? sizeof(float) : sizeof(unsigned short);
     pl->uselen = p.uselen ? p.uselen : h_max_uselen(p.fftlen, p.zmax);
     if (pl->uselen <= 0 || (pl->uselen & 1) || pl->uselen >= 65536 ||
@@ -314,8 +326,20 @@ int plan_geometry(fdas_plan* pl, const long* lengths, int ntrials)
         const double f = (double)pl->cur[h] / (double)pl->mx[h];
         pl->zmax_h[h] = h_calc_required_z(f, p.zmax);
         pl->Z[h] = (pl->zmax_h[h] / DZ) * 2 + 1;
-        pl->L[h] = h_harmonic_length(U, pl->mx[h], pl->cur[h], p.zmax, p.fft_lengths);
+        // the direct correlation's cost does not depend on the FFT length: it keeps
+        // the reference's powers of two for the segment geometry
+        pl->L[h] = h_harmonic_length(U, pl->mx[h], pl->cur[h], p.zmax,
+                                     pl->mma ? FDAS_LENGTHS_REFERENCE : p.fft_lengths);
         pl->off[h] = NUMBETWEEN * h_z_response_halfwidth((double)-pl->zmax_h[h]);
+        if (pl->mma) {
+            int jlo = 0, nk = 0;
+            mma_tile_range(pl->off[h] / NUMBETWEEN, pl->off[h] / NUMBETWEEN, &jlo, &nk);
+            if (nk > kMmaMaxK) {
+                set_error("direct correlation: zmax %d of harmonic %d/%d needs %d K steps (at most "
+                          "%d: zmax up to 300)", pl->zmax_h[h], pl->cur[h], pl->mx[h], nk, kMmaMaxK);
+                return FDAS_ERR_UNSUPPORTED;
+            }
+        }
         pl->zlo[h] = h_calc_required_z(f, -p.zmax);
         pl->seg_off[h] = so;
         so += pl->L[h] / 2;
@@ -451,6 +475,7 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
                     e.eslot = item * H + h;
                     e.nbins = (int)tr.nbins;
                     e.spec_off = tr.spec_off;
+                    e.nch = 0;
                 }
             }
         }
@@ -468,6 +493,26 @@ int plan_tables(fdas_plan* pl, std::vector<int>& fullrlo, std::vector<int>& rlo_
         if (h > 0) sub += (long long)pl->Z[h] * pl->W[h];
     }
     pl->sub_total = sub;
+    if (pl->mma) {
+        // staged chunks per harmonic follow from the final row widths: hi and lo
+        // byte arrays of nch chunks each, harmonics one after another per item
+        long long so = 0;
+        for (int h = 0; h < H; ++h) {
+            mma_geometry(pl->W[h], pl->off[h] / NUMBETWEEN, &pl->nm[h], &pl->nch[h]);
+            pl->seg_off[h] = so;
+            so += 2LL * pl->nch[h];
+        }
+        pl->seg_stride = so;
+        for (size_t c = 0; c < pl->classes.size(); ++c) {
+            const auto& hl = pl->class_harm[c];
+            for (int item = 0; item < S; ++item)
+                for (size_t q = 0; q < hl.size(); ++q) {
+                    SegDesc& e = desc[c][(size_t)item * hl.size() + q];
+                    e.out = (long long)item * pl->seg_stride + pl->seg_off[hl[q]];
+                    e.nch = pl->nch[hl[q]];
+                }
+        }
+    }
     return FDAS_OK;
 }
 
@@ -507,10 +552,11 @@ int plan_device(fdas_plan* pl, bool own_series, const std::vector<int>& fullrlo,
 
     // launch shapes in effect now, and the data layouts they read
     for (int h = 0; h < H; ++h) {
-        pl->corr_var[h] = corr_variant(pl->L[h]);
-        pl->decim[h] = corr_variant_decim(pl->L[h], pl->corr_var[h]);
+        pl->corr_var[h] = pl->mma ? 0 : corr_variant(pl->L[h]);
+        pl->decim[h] = pl->mma ? 0 : corr_variant_decim(pl->L[h], pl->corr_var[h]);
     }
     pl->hs_var = harmsum_variant();
+    pl->mma_var = mma_variant();
     pl->class_decim.assign(pl->classes.size(), 0);
     for (size_t c = 0; c < pl->classes.size(); ++c) pl->class_decim[c] = pl->decim[pl->class_harm[c][0]];
 
@@ -626,8 +672,27 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
             a.prec = pl->prec;
             a.decim = pl->class_decim[c];
             TimedLaunch tl(pl, FDAS_K_SEGPREP);
-            FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
+            if (pl->mma) FDAS_TRY(launch_segment_prep_mma(pl->classes[c], a, pl->stream));
+            else FDAS_TRY(launch_segment_prep(pl->classes[c], a, pl->stream));
         }
+        auto mma_args = [&](int h, void* out, int W) {
+            MmaCorrArgs a{};
+            a.bank = &bs.mma[h];
+            a.seg = pl->d_seg;
+            a.seg_stride = pl->seg_stride;
+            a.seg_off = pl->seg_off[h];
+            a.nch = pl->nch[h];
+            a.seg_inv = reinterpret_cast<const float*>(pl->d_seg_exp);
+            a.H = pl->H;
+            a.h = h;
+            a.nsteps = cs;
+            a.W = W;
+            a.Z = pl->Z[h];
+            a.nm = pl->nm[h];
+            a.out = out;
+            a.variant = pl->mma_var;
+            return a;
+        };
         auto corr_args = [&](int h, void* out, int W) {
             CorrArgs a{};
             a.bank = bs.bankq[h];
@@ -651,7 +716,8 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         for (int h = 1; h < pl->H; ++h) {
             TimedLaunch tl(pl, FDAS_K_CORR_SUB);
             char* sub_h = static_cast<char*>(pl->d_sub) + (size_t)pl->hs.sub_off[h] * pl->pbytes;
-            FDAS_TRY(launch_corr(pl->L[h], corr_args(h, sub_h, pl->W[h]), pl->stream, pl->corr_var[h]));
+            if (pl->mma) FDAS_TRY(launch_corr_mma(mma_args(h, sub_h, pl->W[h]), pl->stream));
+            else FDAS_TRY(launch_corr(pl->L[h], corr_args(h, sub_h, pl->W[h]), pl->stream, pl->corr_var[h]));
         }
         // summed powers for the caller, FP32; with FP32 planes the fundamental
         // plane is that buffer itself and the sum goes back in place
@@ -659,7 +725,8 @@ int run_items(fdas_plan* pl, int item0, int nitems, float* d_planes)
         void* plane = out && pl->prec == FDAS_PREC_FP32 ? static_cast<void*>(out) : pl->d_plane;
         {
             TimedLaunch tl(pl, FDAS_K_CORR_FUND);
-            FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream, pl->corr_var[0]));
+            if (pl->mma) FDAS_TRY(launch_corr_mma(mma_args(0, plane, U), pl->stream));
+            else FDAS_TRY(launch_corr(pl->L[0], corr_args(0, plane, U), pl->stream, pl->corr_var[0]));
         }
         // The harmonic sum as its own tiled kernel: summing inside the fundamental
         // correlation's epilogue (gathering sub-harmonic cells after each inverse

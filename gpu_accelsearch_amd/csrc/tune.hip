@@ -246,6 +246,18 @@ extern "C" int fdas_get_corr_variant(int fftlen) { return corr_variant(fftlen); 
 
 extern "C" int fdas_set_corr_variant(int fftlen, int variant) { return set_corr_variant(fftlen, variant); }
 
+extern "C" int fdas_mma_variant_count(void) { return mma_variant_count(); }
+
+extern "C" int fdas_get_mma_variant(void) { return mma_variant(); }
+
+extern "C" int fdas_set_mma_variant(int variant) { return set_mma_variant(variant); }
+
+extern "C" int fdas_mma_variant_units(int variant) { return mma_variant_units(variant); }
+
+extern "C" int fdas_mma_variant_steps(int variant) { return mma_variant_steps(variant); }
+
+extern "C" int fdas_mma_variant_threads(int variant) { return mma_variant_threads(variant); }
+
 extern "C" int fdas_corr_variant_shape(int fftlen, int variant, int* threads, int* steps_per_wg,
                                        int* lds_form)
 {
@@ -287,9 +299,11 @@ extern "C" int fdas_autotune(const fdas_params* base, const int* fftlens, int nf
     hipStream_t s = nullptr;
     FDAS_HIP(hipStreamCreate(&s));
     int rc = FDAS_OK;
-    // 1. correlation shapes of every FFT length the valid candidates use
+    // 1. correlation shapes of every FFT length the valid candidates use (the
+    //    direct correlation of FDAS_PREC_I16_MMA has no FFT shapes)
     std::vector<int> Ls;
     for (int L0 : cand) {
+        if (base->precision == FDAS_PREC_I16_MMA) break;
         if (!config_valid(L0, base->zmax, base->numharm)) continue;
         const int U = h_max_uselen(L0, base->zmax);
         int cur[MAX_HARM], mx[MAX_HARM];

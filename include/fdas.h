@@ -154,8 +154,15 @@ enum {
 /* Storage of spectra segments and templates (FFT arithmetic, power planes and
  * power accumulation are FP32 in every mode): FP32, FP16, BF16, or I16 (int16
  * pairs: 16-bit block floating point).  Reduced-precision values carry power-of-
- * two exponents per template row and segment. */
-enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2, FDAS_PREC_I16 = 3 };
+ * two exponents per template row and segment.
+ * I16_MMA correlates without FFTs: the normalised segment values and the
+ * templates' time-domain taps are quantised to integers of at most 32639 in
+ * magnitude (one scale per segment and per template row), and each kept column
+ * is an exact integer sum computed on the matrix cores (int8 halves, 32-bit
+ * accumulation), scaled back to an FP32 power.  It uses the reference's
+ * power-of-two FFT lengths for the segment geometry and supports zmax <= 300. */
+enum { FDAS_PREC_FP32 = 0, FDAS_PREC_FP16 = 1, FDAS_PREC_BF16 = 2, FDAS_PREC_I16 = 3,
+       FDAS_PREC_I16_MMA = 4 };
 
 #define FDAS_MAX_HARM 16
 #define FDAS_MAX_STAGES 5
@@ -412,6 +419,16 @@ int fdas_get_harmsum_variant(void);
 int fdas_set_harmsum_variant(int variant);
 int fdas_get_corr_variant(int fftlen);
 int fdas_set_corr_variant(int fftlen, int variant);
+/* Direct correlation (FDAS_PREC_I16_MMA): variant v keeps
+ * fdas_mma_variant_units(v) M tiles (256 output pairs of two templates each) per
+ * wavefront in flight, and a workgroup of fdas_mma_variant_threads(v) threads
+ * takes fdas_mma_variant_steps(v) steps of one template tile. */
+int fdas_mma_variant_count(void);
+int fdas_get_mma_variant(void);
+int fdas_set_mma_variant(int variant);
+int fdas_mma_variant_units(int variant);
+int fdas_mma_variant_steps(int variant);
+int fdas_mma_variant_threads(int variant);
 /* Shape of a correlation variant: threads, steps per workgroup and LDS exchange
  * form (as fdas_tune_report.corr_form; 1 at a power of two is the split form). */
 int fdas_corr_variant_shape(int fftlen, int variant, int *threads, int *steps_per_wg,

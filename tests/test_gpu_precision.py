@@ -4,7 +4,9 @@ two exponents, and the power planes in 16 bits (P16); FFT arithmetic and power
 accumulation stay FP32.
 
 The power tolerance for reduced precision is 1e-3 of max(P_ref, 1).  I16 meets
-it and is tested against it.  FP16 and BF16 do not: their 11- and 8-bit
+it and is tested against it, as is I16_MMA (the same 16-bit quantisation, time-
+domain, correlated exactly in integers on the matrix cores; tests/test_gpu_mma.py
+covers its other shapes).  FP16 and BF16 do not: their 11- and 8-bit
 significands round every stored template and segment value, and that rounding
 adds up in each correlation output.  Their tests assert the stated limit of
 each mode: the largest error measured over these workloads, with a small
@@ -17,8 +19,9 @@ from conftest import print_case
 
 pytestmark = pytest.mark.gpu
 
-# I16: the 1e-3 tolerance (measured at most 5.7e-4 over these tests).
-IN_TOL = {"i16": 1e-3}
+# I16 and I16_MMA: the 1e-3 tolerance (measured at most 5.7e-4 and 6.6e-4 over
+# these tests and tests/test_gpu_mma.py).
+IN_TOL = {"i16": 1e-3, "i16mma": 1e-3}
 # Stated limits: largest |dP| / max(P_ref, 1) measured over these tests, plus a
 # small margin.  FP16 1.6e-3, BF16 1.27e-2.
 STATED_LIMIT = {"fp16": 1.8e-3, "bf16": 1.4e-2}
@@ -164,7 +167,7 @@ def _workload(fd, c4, mode):
     assert len(want) > 0 and agree
 
 
-@pytest.mark.parametrize("mode", ["fp32", "i16"])
+@pytest.mark.parametrize("mode", ["fp32", "i16", "i16mma"])
 def test_precision_workload_in_tolerance(fd, c4, mode):
     _workload(fd, c4, mode)
 

@@ -35,7 +35,7 @@ def test_abi_exports_every_declared_symbol():
 @pytest.mark.parametrize("kw,code", [
     (dict(n=1001), 4), (dict(n=1000), 1), (dict(zmax=201), 1), (dict(zmax=1300), 1), (dict(numharm=3), 1),
     (dict(fftlen=512), 4), (dict(fftlen=6000), 4), (dict(uselen=7735), 1),
-    (dict(uselen=8000), 1),
+    (dict(uselen=8000), 1), (dict(zmax=400, precision="i16mma"), 4),
 ])
 def test_plan_rejects_bad_parameters(kw, code):
     import gpu_accelsearch_amd as fd
