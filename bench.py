@@ -31,8 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12        # bytes/s, MI355X HBM3E
-VALU_PEAK = 157.3e12     # FP32 vector FLOP/s (counts packed FP32 instructions)
-VALU_UNPACKED = 78.65e12  # FP32 vector FLOP/s without packed instructions (these kernels)
+VALU_PEAK = 157.3e12     # FP32 vector FLOP/s, MI355X published peak
 POWCUT = (40.0, 45.0, 55.0, 70.0, 0.0)
 
 
@@ -320,7 +319,7 @@ def main() -> int:
                     help="contiguous steps of the FP64 oracle in the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "i16"],
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp16", "bf16", "i16", "i16mma"],
                     help="storage of the single-trial form (arithmetic is FP32)")
     ap.add_argument("--batch-trials", type=int, default=8,
                     help="DM trials of the batched forms (0: skip)")
@@ -420,7 +419,7 @@ def main() -> int:
         kernel_rooflines[cat] = {
             "bound": "valu", "kernel": kname, "launches_per_trial": launches,
             "avg_launch_ms": ms, "flops_per_launch": flops, "achieved_TFLOPs": rate / 1e12,
-            "frac": rate / VALU_PEAK, "frac_of_unpacked_78.6": rate / VALU_UNPACKED,
+            "frac": rate / VALU_PEAK,
             "hbm_GBps": by / (ms * 1e-3) / 1e9, "hbm_frac": by / (ms * 1e-3) / HBM_PEAK}
     rf = kernel_rooflines["harmsum"]
     out = {
@@ -474,8 +473,13 @@ def main() -> int:
         bi["fftlen_ms_per_trial"] = bt["ms_per_trial"]
         bi["storage"] = "I16 segments and templates, P16 power planes (within the 1e-3 tolerance)"
         bf = batch_form(series, n, zmax, numharm, args.batch_trials, "fp32", fftlen)
+        # the direct correlation on the matrix cores, same batch and length
+        bm = batch_form(series, n, zmax, numharm, args.batch_trials, "i16mma", fftlen)
+        bm["storage"] = ("I16_MMA: time-domain 15-bit segments and templates, exact int8-MFMA "
+                         "correlation, P16 power planes (within the 1e-3 tolerance)")
         out["batch_i16"] = bi
         out["batch_fp32"] = bf
+        out["batch_i16mma"] = bm
         out["ratio_batched_reduced_vs_fused_fp32_single"] = bi["dm_trials_per_s"] * per_trial
         out["ratio_batched_reduced_vs_batched_fp32"] = bi["dm_trials_per_s"] / bf["dm_trials_per_s"]
     if args.stage_steps > 0:
