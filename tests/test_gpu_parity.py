@@ -168,13 +168,15 @@ def test_plan_uneven_length(fd, oracle):
 
 
 def test_stage_pipeline_matches_plan(fd, oracle):
-    """Reference-named stage entry points (AccelSearch.cu:217-241) on one step."""
+    """Reference-named stage entry points (AccelSearch.cu:217-241) on one step: the
+    summed plane against the FP64 oracle (1e-4) and against the fused plan."""
     from gpu_accelsearch_amd import synth
     n = 1 << 20
     x = synth.plain_series(n, seed=3, r0=70000.2, z0=-30.0)
     zmax, numharm, fftlen = 50, 4, 2048
     print_case("stage pipeline")
     P = fd.SearchParams(n=n, zmax=zmax, numharm=numharm, fftlen=fftlen)
+    ref = oracle.Search(x, n, 6.4e-5, zmax, numharm, fftlen)
     with fd.AccelSearch(P) as s:
         s.prepare(x)
         spec = s.spectrum()
@@ -182,15 +184,19 @@ def test_stage_pipeline_matches_plan(fd, oracle):
         try:
             bank = fd.createSubharmonicInfo(3, zmax)
             pipe = fd.StagePipeline(bank)
-            for st in (0, 11, s.nsteps - 1):
+            for st in (0, 11, synth.steps_covering(70000.2, s.rlo, s.uselen), s.nsteps - 1):
                 startr = s.rlo + st * (s.uselen // 2)
                 got = pipe.step(spec, startr, -zmax, s.uselen, s.nbins)
                 want = s.run_steps(st, 1)[0]
                 assert power_err(got, want.astype(np.float64)) <= 2e-5
+                err = power_err(got, ref.step_plane(st))
+                print(f"stage pipeline step {st}: max |dP|/max(P,1) vs oracle {err:.2e}")
+                assert err <= 1e-4, err
             pipe.close()
             bank.free()
         finally:
             fd.accel.fdas_set_uselen(32000)
+            ref.close()
 
 
 def test_candidates_match_oracle(fd, oracle):
