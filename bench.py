@@ -410,8 +410,8 @@ def main() -> int:
         "avg_launch_ms": hs_ms, "alg_bytes_per_launch": hs_bytes,
         "achieved_GBps": hs_bytes / (hs_ms * 1e-3) / 1e9,
         "frac": hs_bytes / (hs_ms * 1e-3) / HBM_PEAK}}
-    for cat, kname in (("corr_fund", "corr_sub_kernel (fundamental)"),
-                       ("corr_sub", "corr_sub/corr_odd_kernel (sub-harmonics)")):
+    for cat, kname in (("corr_fund", "corr_wave_kernel (fundamental)"),
+                       ("corr_sub", "corr_odd/corr_wave_kernel (sub-harmonics)")):
         launches, ms = per_launch(cat)
         flops = fl[cat] / launches
         by = ab[cat] / launches

@@ -5,6 +5,7 @@
 // Built by tests/test_fft_host_sim.py with g++ and called through ctypes.
 #include "../gpu_accelsearch_amd/csrc/fft_block.hpp"
 
+#include <utility>
 #include <vector>
 
 using namespace fdas;
@@ -127,9 +128,40 @@ extern "C" int fft_odd_sim(int r, int m, int t, int dir, const float* in, float*
     return -1;
 }
 
+// Lane-swap policy that records the register pairs BlockFFT::lane_exchange_last
+// exchanges (the sequence is the same in every lane); apply_lane_swaps then makes
+// each exchange across the 64 lanes of every wavefront at once, with the device's
+// semantics (lane_swap): the lane with the bit set takes the partner's y into x,
+// the lane with it clear the partner's x into y.
+struct SwapOp {
+    int lb, xi, yi;
+};
+struct RecordSwap {
+    std::vector<SwapOp>* ops;
+    const cplx* base;
+    template <int LB>
+    void swap(cplx& x, cplx& y) const
+    {
+        ops->push_back({LB, (int)(&x - base), (int)(&y - base)});
+    }
+};
+
+template <int E>
+static void apply_lane_swaps(const std::vector<SwapOp>& ops, std::vector<cplx>& v, int T)
+{
+    for (const SwapOp& op : ops)
+        for (int t = 0; t < T; ++t)
+            if (t & (1 << op.lb)) {
+                const int p = t ^ (1 << op.lb);
+                std::swap(v[(size_t)t * E + op.xi], v[(size_t)p * E + op.yi]);
+            }
+}
+
 // Wavefront-split transform (WaveFFT): inputs in the decimated order
 // x[w + W (lane + 64 i)], outputs in natural order; each phase for every thread in
-// turn (the wavefront passes, the cross-wavefront store, then the loads).
+// turn (the wavefront passes, the cross-wavefront store, then the loads).  As on
+// the device, the last wavefront exchange is made in registers where
+// FW::LANE_LAST holds.
 template <int N, int T, int DIR, int P>
 static void run_rest_wave(std::vector<cplx>& v, std::vector<cplx>& set, const std::vector<cplx>& tw)
 {
@@ -137,10 +169,17 @@ static void run_rest_wave(std::vector<cplx>& v, std::vector<cplx>& set, const st
     using FW = typename WF::FW;
     constexpr int E = WF::E;
     if constexpr (P < FW::NPASS) {
-        for (int t = 0; t < T; ++t)
-            FW::template store<P - 1>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
-        for (int t = 0; t < T; ++t)
-            FW::template load<P>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
+        if constexpr (FW::LANE_LAST && P == FW::NPASS - 1) {
+            std::vector<SwapOp> ops;
+            cplx probe[E];
+            FW::lane_exchange_last(probe, RecordSwap{&ops, probe});
+            apply_lane_swaps<E>(ops, v, T);
+        } else {
+            for (int t = 0; t < T; ++t)
+                FW::template store<P - 1>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
+            for (int t = 0; t < T; ++t)
+                FW::template load<P>(&v[(size_t)t * E], set.data() + (t >> 6) * WF::BAND, t & 63);
+        }
         for (int t = 0; t < T; ++t) FW::template compute<P>(&v[(size_t)t * E], t & 63, tw.data());
         run_rest_wave<N, T, DIR, P + 1>(v, set, tw);
     }

@@ -68,6 +68,24 @@ void host_wave_barrier(int)
     }
 }
 
+// lane_swap for the calling thread (lane l of its wavefront) through a buffer its
+// wavefront shares: every lane publishes its pair, then the lane with bit lb set
+// takes the partner's y into x and the lane with it clear the partner's x into y.
+namespace {
+cplx g_lane_pair[16][64][2];
+}  // namespace
+
+void host_lane_swap(int lb, cplx& x, cplx& y)
+{
+    const int w = g_tid >> 6, l = g_tid & 63, p = l ^ (1 << lb);
+    g_lane_pair[w][l][0] = x;
+    g_lane_pair[w][l][1] = y;
+    host_wave_barrier(g_tid);
+    if (l & (1 << lb)) x = g_lane_pair[w][p][1];
+    else y = g_lane_pair[w][p][0];
+    host_wave_barrier(g_tid);
+}
+
 }  // namespace fdas
 
 using namespace fdas;
