@@ -555,8 +555,11 @@ FDAS_SHAPES(2048, {256, 8, 0}, {128, 8, 0}, {128, 8, 1}, {128, 16, 0}, {256, 8, 
 // 256 steps per launch); slower at 2048.
 FDAS_SHAPES(4096, {512, 16, 8}, {512, 8, 2}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {256, 8, 0},
             {512, 4, 0}, {256, 8, 1}, {128, 8, 1}, {512, 8, 8}, {256, 8, 8})
+// At 8192, 32 steps per workgroup in the wavefront-split form (last variant) measured 2-3%
+// faster on a 1084-step trial, but a short trial gets half the workgroups (135 steps of
+// 56 rows: 280 workgroups, 1.1 per CU), so it is left to the tuner.
 FDAS_SHAPES(8192, {512, 16, 8}, {512, 16, 2}, {512, 16, 0}, {512, 8, 0}, {512, 32, 0}, {1024, 8, 0},
-            {512, 8, 1}, {256, 8, 1}, {512, 8, 2}, {512, 8, 8})
+            {512, 8, 1}, {256, 8, 1}, {512, 8, 2}, {512, 8, 8}, {512, 32, 8})
 FDAS_SHAPES(16384, {512, 1, 0}, {1024, 1, 0}, {512, 1, 1}, {1024, 4, 1})
 // Lengths R*2^k (R = 3, 5, 7): R transforms of L/R per step and no split form; S = 1
 // runs their LDS exchanges one transform after another through one region (1/R
