@@ -121,6 +121,30 @@ struct PStore<unsigned short> {
     }
 };
 
+// Stores of one power-plane row of W elements through a buffer resource that covers
+// exactly the row: the hardware drops a lane whose column is outside [0, W) (a
+// negative column wraps to a large unsigned offset), so a kernel stores every one of
+// its values without a branch, and the number of stores it has in flight is the same
+// on every path (which lets the compiler's memory-wait counting keep loads issued
+// before them in flight).  The descriptor's fourth word is the gfx9 one for raw
+// (untyped) buffer access.
+template <typename OutE>
+struct RowStore {
+    __amdgpu_buffer_rsrc_t r;
+    __device__ __forceinline__ RowStore(OutE* row, int W)
+        : r(__builtin_amdgcn_make_buffer_rsrc(row, (short)0, W * (int)sizeof(OutE), 0x00020000))
+    {
+    }
+    __device__ __forceinline__ void put(int col, float power) const
+    {
+        const OutE e = PStore<OutE>::put(power);
+        if constexpr (sizeof(OutE) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(e), r, col * 4, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b16(e, r, col * 2, 0, 0);
+    }
+};
+
 // Plane element of storage mode P.
 template <int P>
 using PlaneT = typename std::conditional<P == PREC_F32, float, unsigned short>::type;

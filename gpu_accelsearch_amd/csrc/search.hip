@@ -274,22 +274,29 @@ __global__ __launch_bounds__(T) void corr_wave_kernel(
 #pragma unroll
     for (int i = 0; i < E; ++i) kr[i] = K[64 * i];
     const int ek = P == PREC_F32 ? 0 : bank_exp[z];
+    // A step's segment values are loaded one step ahead, during the previous step's
+    // transform (the last step reloads its own, unused), and the powers go out
+    // through row-bounded buffer stores (RowStore), so a step waits on no memory
+    // access of its own.
+    const S* Aw = seg + seg_off + wv * (NW / 2) + lane;
+    S an[EH];
+#pragma unroll
+    for (int i = 0; i < EH; ++i) an[i] = Aw[s0 * seg_stride + 64 * i];
     for (int s = s0; s < s1; ++s) {
-        const S* A = seg + s * seg_stride + seg_off + wv * (NW / 2) + lane;
         cplx a[EH], v[E];
 #pragma unroll
-        for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(A[64 * i]);
+        for (int i = 0; i < EH; ++i) a[i] = CStore<P>::get(an[i]);
+        const long long sn = min(s + 1, s1 - 1);
+#pragma unroll
+        for (int i = 0; i < EH; ++i) an[i] = Aw[sn * seg_stride + 64 * i];
 #pragma unroll
         for (int i = 0; i < E; ++i) v[i] = cmulc(a[i % EH], CStore<P>::get(kr[i]));
         F::run(v, lds + ((s - s0) & 1) * F::SET, tw, t);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
-        OutE* o = out + ((size_t)s * Z + z) * W;
+        const RowStore<OutE> o(out + ((size_t)s * Z + z) * W, W);
 #pragma unroll
-        for (int i = 0; i < E; ++i) {
-            const int col = F::out_index(t, i) - off;
-            if (col >= 0 && col < W)
-                o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
-        }
+        for (int i = 0; i < E; ++i)
+            o.put(F::out_index(t, i) - off, (v[i].x * v[i].x + v[i].y * v[i].y) * sc);
     }
 }
 
