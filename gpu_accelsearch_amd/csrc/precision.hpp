@@ -122,12 +122,20 @@ struct PStore<unsigned short> {
 };
 
 // Stores of one power-plane row of W elements through a buffer resource that covers
-// exactly the row: the hardware drops a lane whose column is outside [0, W) (a
-// negative column wraps to a large unsigned offset), so a kernel stores every one of
-// its values without a branch, and the number of stores it has in flight is the same
-// on every path (which lets the compiler's memory-wait counting keep loads issued
-// before them in flight).  The descriptor's fourth word is the gfx9 one for raw
-// (untyped) buffer access.
+// exactly the row: the hardware's range check drops a store at an offset of W
+// elements or more, so a kernel stores every one of its values without a branch, and
+// the number of stores it has in flight is the same on every path (which lets the
+// compiler's memory-wait counting keep loads issued before them in flight).  The
+// byte offset of column col is col * sizeof(OutE): at least W elements for col >= W,
+// and for a negative column (at least -2^16) a wrapped value near 2^32, far out of
+// range too.  An offset must reach the store whole: the compiler may split one such
+// as (base + constant) * size into a register part and an immediate, and a wrapped
+// (negative) register part plus an immediate is not brought back into range by the
+// hardware, which then drops in-range stores.  put() hides the offset's form behind
+// an empty asm; put_linear() leaves it visible (measured 2-5% faster in the
+// wavefront-split correlation's step loop), and tests/test_host_logic.py checks
+// the built library for power stores with an immediate offset.  The descriptor's
+// fourth word is the gfx9 one for raw (untyped) buffer access.
 template <typename OutE>
 struct RowStore {
     __amdgpu_buffer_rsrc_t r;
@@ -135,13 +143,23 @@ struct RowStore {
         : r(__builtin_amdgcn_make_buffer_rsrc(row, (short)0, W * (int)sizeof(OutE), 0x00020000))
     {
     }
-    __device__ __forceinline__ void put(int col, float power) const
+    __device__ __forceinline__ void put_at(unsigned off, float power) const
     {
         const OutE e = PStore<OutE>::put(power);
         if constexpr (sizeof(OutE) == 4)
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(e), r, col * 4, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(e), r, (int)off, 0, 0);
         else
-            __builtin_amdgcn_raw_buffer_store_b16(e, r, col * 2, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(e, r, (int)off, 0, 0);
+    }
+    __device__ __forceinline__ void put(int col, float power) const
+    {
+        unsigned o = (unsigned)col * (unsigned)sizeof(OutE);
+        asm("" : "+v"(o));   // no side effect: may be hoisted, never seen through
+        put_at(o, power);
+    }
+    __device__ __forceinline__ void put_linear(int col, float power) const
+    {
+        put_at((unsigned)col * (unsigned)sizeof(OutE), power);
     }
 };
 

@@ -234,13 +234,10 @@ __global__ __launch_bounds__(T) void corr_sub_kernel(
             corr_ifft<L, T, P, NR>(k1, seg + s * seg_stride + seg_off, tw, lds, v, t);
         }
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
-        OutE* o = out + ((size_t)s * Z + z) * W;
+        const RowStore<OutE> o(out + ((size_t)s * Z + z) * W, W);
 #pragma unroll
-        for (int i = 0; i < F::E; ++i) {
-            const int col = F::out_index(t, i) - off;
-            if (col >= 0 && col < W)
-                o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
-        }
+        for (int i = 0; i < F::E; ++i)
+            o.put(F::out_index(t, i) - off, (v[i].x * v[i].x + v[i].y * v[i].y) * sc);
     }
 }
 
@@ -296,7 +293,7 @@ __global__ __launch_bounds__(T) void corr_wave_kernel(
         const RowStore<OutE> o(out + ((size_t)s * Z + z) * W, W);
 #pragma unroll
         for (int i = 0; i < E; ++i)
-            o.put(F::out_index(t, i) - off, (v[i].x * v[i].x + v[i].y * v[i].y) * sc);
+            o.put_linear(F::out_index(t, i) - off, (v[i].x * v[i].x + v[i].y * v[i].y) * sc);
     }
 }
 
@@ -351,15 +348,12 @@ __global__ __launch_bounds__(T) void corr_odd_kernel(
         else F::template run_multi<R>(v, lds, tw, t);
         RadixOdd<R, M, T>::inv_combine(v, t, twr);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
-        OutE* o = out + ((size_t)s * Z + z) * W;
+        const RowStore<OutE> o(out + ((size_t)s * Z + z) * W, W);
 #pragma unroll
         for (int q = 0; q < R; ++q)
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                const int col = F::out_index(t, i) + M * q - off;
-                if (col >= 0 && col < W)
-                    o[col] = PStore<OutE>::put((v[q][i].x * v[q][i].x + v[q][i].y * v[q][i].y) * sc);
-            }
+            for (int i = 0; i < E; ++i)
+                o.put(F::out_index(t, i) + M * q - off, (v[q][i].x * v[q][i].x + v[q][i].y * v[q][i].y) * sc);
     }
 }
 
@@ -403,7 +397,7 @@ __global__ __launch_bounds__(T) void corr_split_kernel(
 #pragma unroll
         for (int i = 0; i < E; ++i) a[i] = CStore<P>::get(A[F::in_index(t, i)]);
         const float sc = P == PREC_F32 ? scale : ldexpf(scale, 2 * (ek + seg_exp[s * H + h]));
-        OutE* o = out + ((size_t)s * Z + z) * W;
+        const RowStore<OutE> o(out + ((size_t)s * Z + z) * W, W);
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
 #pragma unroll
@@ -415,11 +409,8 @@ __global__ __launch_bounds__(T) void corr_split_kernel(
             }
             F::run(v, lds, tw, t);
 #pragma unroll
-            for (int i = 0; i < E; ++i) {
-                const int col = 2 * F::out_index(t, i) + half - off;
-                if (col >= 0 && col < W)
-                    o[col] = PStore<OutE>::put((v[i].x * v[i].x + v[i].y * v[i].y) * sc);
-            }
+            for (int i = 0; i < E; ++i)
+                o.put(2 * F::out_index(t, i) + half - off, (v[i].x * v[i].x + v[i].y * v[i].y) * sc);
         }
     }
 }

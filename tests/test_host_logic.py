@@ -32,6 +32,49 @@ def test_abi_exports_every_declared_symbol():
     assert set(names) <= bound, set(names) - bound
 
 
+LLVM_BIN = "/opt/rocm/llvm/bin"
+
+
+def test_power_stores_carry_no_immediate_offset(tmp_path):
+    """The correlation kernels store powers through row-bounded buffer stores whose
+    byte offset is col * size, a wrapped (huge) value for a negative column.  If the
+    compiler split such an offset into a register part and an immediate, a negative
+    register part plus the immediate would not wrap back into range in the
+    hardware's check, and in-range powers would be dropped.  Every correlation
+    kernel of the built library (gfx950 code object, disassembled) must store with
+    a whole register offset."""
+    lib = os.path.join(ROOT, "gpu_accelsearch_amd", "libfdas.so")
+    tools = [os.path.join(LLVM_BIN, t) for t in ("llvm-objcopy", "clang-offload-bundler", "llvm-objdump")]
+    if not os.path.exists(lib) or not all(os.path.exists(t) for t in tools):
+        pytest.skip("needs the built library and the ROCm LLVM tools")
+    fat = tmp_path / "fatbin"
+    subprocess.run([tools[0], f"--dump-section=.hip_fatbin={fat}", lib, str(tmp_path / "copy.so")],
+                   check=True)
+    data = fat.read_bytes()
+    starts = [m.start() for m in re.finditer(re.escape(b"__CLANG_OFFLOAD_BUNDLE__"), data)]
+    stores = immediate = 0
+    kernels = set()
+    for k, p in enumerate(starts):
+        part = tmp_path / f"bundle{k}"
+        part.write_bytes(data[p:starts[k + 1] if k + 1 < len(starts) else len(data)])
+        obj = tmp_path / f"gfx950_{k}.o"
+        subprocess.run([tools[1], "--unbundle", "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950",
+                        f"--input={part}", f"--output={obj}"], check=True)
+        asm = subprocess.run([tools[2], "-d", "--mcpu=gfx950", str(obj)], check=True,
+                             capture_output=True, text=True).stdout
+        for fn in re.split(r"\n(?=[0-9a-f]{16} <)", asm):
+            m = re.match(r"[0-9a-f]{16} <([^>]+)>:", fn)
+            if not m or not re.search(r"corr_(wave|odd|sub|split)_kernel", m.group(1)):
+                continue
+            kernels.add(m.group(1))
+            for line in fn.splitlines():
+                if "buffer_store" in line:
+                    stores += 1
+                    immediate += bool(re.search(r"offset:\s*\d", line))
+    assert len(kernels) > 100 and stores > 1000, (len(kernels), stores)
+    assert immediate == 0, f"{immediate} of {stores} power stores carry an immediate offset"
+
+
 @pytest.mark.parametrize("kw,code", [
     (dict(n=1001), 4), (dict(n=1000), 1), (dict(zmax=201), 1), (dict(zmax=1300), 1), (dict(numharm=3), 1),
     (dict(fftlen=512), 4), (dict(fftlen=6000), 4), (dict(uselen=7735), 1),
