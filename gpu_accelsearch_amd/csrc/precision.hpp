@@ -115,8 +115,12 @@ struct PStore<unsigned short> {
     }
     __device__ __forceinline__ static unsigned short put(float v)
     {
+        // Round to nearest even at bit 13 and remove the bias in one signed sum before
+        // an arithmetic shift ((a - b 2^13) >> 13 == (a >> 13) - b exactly), so the
+        // encoding is four instructions; the same code as shifting first for every
+        // non-negative v, +inf and NaN included.
         const unsigned u = __float_as_uint(v);
-        const int q = (int)((u + 0xfffu + ((u >> 13) & 1u)) >> 13) - (int)(kBias << 10);
+        const int q = (int)(u + 0xfffu + ((u >> 13) & 1u) - (kBias << 23)) >> 13;
         return (unsigned short)min(max(q, 0), 65535);
     }
 };
